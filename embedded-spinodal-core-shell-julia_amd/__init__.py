@@ -1,0 +1,13 @@
+"""MI355X (gfx950) core-shell Cahn-Hilliard step library: host package.
+
+The compute path is ``libcsg.so`` (HIP kernels behind the C ABI of
+``include/csg.h``); this package binds it with ctypes and mirrors the API of
+``core_shell.jl`` (``Parameters``, ``make_grid``, ``initial_fields``,
+``geometry``, ``bulk_energy``, ``bulk_derivative``, ``chemical_potential``,
+``transport_rate``, ``step``, ``energy``, ``run_simulation``).
+"""
+from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, Shape,  # noqa: F401
+                         bulk_derivative, bulk_energy, chemical_potential, energy, geometry,
+                         initial_fields, make_grid, run_simulation, step, transport_rate)
+from ._capi import CsgError, lib  # noqa: F401
+from .writers import julia_float, writedlm  # noqa: F401

@@ -1,0 +1,278 @@
+"""Host API of the HIP core-shell step library, shaped like ``core_shell.jl``.
+
+Each function keeps the reference's name, argument meaning and error
+behaviour (``core_shell.jl:4-103``); the arithmetic runs in ``libcsg.so`` on
+the GPU (there is no CPU path).  Arrays are indexed like the Julia matrices,
+``a[i, j]`` with ``i`` the first index (along which ``k_x`` varies), and are
+handed to the C ABI in column-major order.
+
+Extensions beyond the reference signature, all keyword-only with defaults that
+reproduce the reference: ``precision`` ("fp64" or "mixed"), ``history_every``
+(1 = every step, as ``core_shell.jl:89-92``) and ``device``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import weakref
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib, ptr
+
+
+@dataclasses.dataclass(frozen=True)
+class Parameters:
+    """``Parameters`` (core_shell.jl:4-15)."""
+    A: float = 2.0
+    B: float = 4.0
+    chi: float = 0.5
+    P: float = 5.0
+    ca: float = 0.5
+    cb: float = 0.0
+    cc: float = 1.0
+    kappa: float = 2.0
+    mobility: float = 1.0
+    Q: float = 0.5
+
+    def _c(self) -> _capi.CsgParams:
+        return _capi.CsgParams(*(float(getattr(self, f.name)) for f in dataclasses.fields(self)))
+
+
+class NonFiniteCompositionError(RuntimeError):
+    """Raised as ``error("Non-finite composition at step $s")`` (core_shell.jl:93)."""
+
+    def __init__(self, step: int):
+        super().__init__(f"Non-finite composition at step {step}")
+        self.step = step
+
+
+@dataclasses.dataclass(frozen=True)
+class Grid:
+    """``make_grid`` result (core_shell.jl:17-25): ``kx`` (n x 1) and ``ky``
+    (1 x n) with the Nyquist mode zeroed, ``k2`` (n x n) with it kept.  The
+    device computes wavenumbers on the fly; these arrays are for callers."""
+    kx: np.ndarray
+    ky: np.ndarray
+    k2: np.ndarray
+    spacing: float
+
+    @property
+    def n(self) -> int:
+        return self.k2.shape[0]
+
+
+def _fftfreq(n: int, fs: float) -> np.ndarray:
+    m = np.arange(n, dtype=np.float64)
+    m[m >= (n + 1) // 2] -= n
+    return m * (fs / n)
+
+
+def make_grid(n: int, spacing: float) -> Grid:
+    k = (2 * np.pi) * _fftfreq(n, 1 / spacing)
+    dk = k.copy()
+    if n % 2 == 0:
+        dk[n // 2] = 0.0
+    return Grid(dk.reshape(n, 1), dk.reshape(1, n), (k ** 2).reshape(n, 1) + (k ** 2).reshape(1, n),
+                float(spacing))
+
+
+def _fortran(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def initial_fields(n: int, spacing: float = 1.0, radius: float = 70.0, width: float = 4.0,
+                   seed: int = 5749):
+    """``initial_fields`` (core_shell.jl:27-37), noise bit-identical to Julia's
+    ``rand(MersenneTwister(seed), n, n)``."""
+    c = np.empty((n, n), order="F")
+    phi = np.empty((n, n), order="F")
+    rc = lib().csg_initial_fields(int(n), float(spacing), float(radius), float(width), int(seed),
+                                  ptr(c), ptr(phi))
+    check(rc)
+    return c, phi
+
+
+_PREC = {"fp64": _capi.CSG_FP64, "mixed": _capi.CSG_MIXED}
+
+
+class Session:
+    """One device context (grid size, spacing, parameters, precision)."""
+
+    def __init__(self, n: int, spacing: float = 1.0, p: Parameters = Parameters(),
+                 precision: str = "fp64", device: int = 0):
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        self.n, self.spacing, self.p, self.precision = int(n), float(spacing), p, precision
+        h = _capi._VP()
+        pc = p._c()
+        check(lib().csg_create(h, self.n, self.spacing, pc, _PREC[precision], int(device)))
+        self.handle = h
+        self._phi_key = None
+        self._fin = weakref.finalize(self, lib().csg_destroy, h)
+
+    def close(self):
+        self._fin()
+
+    def set_fields(self, c0, phi):
+        c0, phi = _fortran(c0), _fortran(phi)
+        self._check_shape(c0)
+        self._check_shape(phi)
+        check(lib().csg_set_fields(self.handle, ptr(c0), ptr(phi)))
+        self._phi_key = _key(phi)
+
+    def ensure_phi(self, phi):
+        phi = _fortran(phi)
+        k = _key(phi)
+        if k != self._phi_key:
+            self.set_fields(phi, phi)
+
+    def _check_shape(self, a):
+        if a.shape != (self.n, self.n):
+            raise ValueError(f"expected a {self.n}x{self.n} array, got {a.shape}")
+
+    def run(self, dt: float, steps: int, history_every: int = 1) -> np.ndarray:
+        rows = steps // history_every + 1
+        hist = np.zeros((rows, 6), order="F")
+        rc = lib().csg_run(self.handle, float(dt), int(steps), int(history_every), ptr(hist))
+        if rc == _capi.CSG_ERR_NONFINITE:
+            raise NonFiniteCompositionError(int(lib().csg_nonfinite_step(self.handle)))
+        check(rc)
+        return hist
+
+    def step(self, dt: float, nsteps: int = 1):
+        check(lib().csg_step(self.handle, float(dt), int(nsteps)))
+
+    def field(self) -> np.ndarray:
+        c = np.empty((self.n, self.n), order="F")
+        check(lib().csg_get_field(self.handle, ptr(c)))
+        return c
+
+    def time_passes(self, dt: float, nsteps: int) -> np.ndarray:
+        ms = np.zeros(5)
+        check(lib().csg_time_passes(self.handle, float(dt), int(nsteps), ptr(ms)))
+        return ms
+
+    def device_bytes(self) -> int:
+        return int(lib().csg_device_bytes(self.handle))
+
+
+def _key(phi: np.ndarray):
+    return (phi.shape, hash(phi.tobytes()))
+
+
+_sessions: dict = {}
+
+
+def _session(n, spacing, p, precision, device=0) -> Session:
+    k = (int(n), float(spacing), p, precision, int(device))
+    s = _sessions.get(k)
+    if s is None:
+        s = Session(n, spacing, p, precision, device)
+        _sessions[k] = s
+    return s
+
+
+@dataclasses.dataclass(frozen=True)
+class Shape:
+    """``geometry`` result (core_shell.jl:39-43): h, g, M and the phi they came from."""
+    h: np.ndarray
+    g: np.ndarray
+    M: np.ndarray
+    phi: np.ndarray
+
+
+def geometry(phi, p: Parameters = Parameters(), *, precision: str = "fp64", device: int = 0) -> Shape:
+    phi = _fortran(phi)
+    s = _session(phi.shape[0], 1.0, p, precision, device)
+    s.ensure_phi(phi)
+    n = phi.shape[0]
+    h, g, M = (np.empty((n, n), order="F") for _ in range(3))
+    check(lib().csg_geometry(s.handle, ptr(h), ptr(g), ptr(M)))
+    return Shape(h, g, M, phi)
+
+
+def _ctx(shape: Shape, spacing: float, p: Parameters, precision: str, device: int) -> Session:
+    s = _session(shape.phi.shape[0], spacing, p, precision, device)
+    s.ensure_phi(shape.phi)
+    return s
+
+
+def bulk_energy(c, shape: Shape, p: Parameters = Parameters(), *, precision="fp64", device=0):
+    """``bulk_energy`` (core_shell.jl:45-49)."""
+    s = _ctx(shape, 1.0, p, precision, device)
+    c = _fortran(c)
+    out = np.empty_like(c, order="F")
+    check(lib().csg_bulk_energy(s.handle, ptr(c), ptr(out)))
+    return out
+
+
+def bulk_derivative(c, shape: Shape, p: Parameters = Parameters(), *, precision="fp64", device=0):
+    """``bulk_derivative`` (core_shell.jl:51-55)."""
+    s = _ctx(shape, 1.0, p, precision, device)
+    c = _fortran(c)
+    out = np.empty_like(c, order="F")
+    check(lib().csg_bulk_derivative(s.handle, ptr(c), ptr(out)))
+    return out
+
+
+def chemical_potential(c, shape: Shape, grid: Grid, p: Parameters = Parameters(), *,
+                       precision="fp64", device=0):
+    """``chemical_potential`` (core_shell.jl:57-60)."""
+    s = _ctx(shape, grid.spacing, p, precision, device)
+    c = _fortran(c)
+    mu = np.empty_like(c, order="F")
+    check(lib().csg_chemical_potential(s.handle, ptr(c), ptr(mu)))
+    return mu
+
+
+def transport_rate(c, shape: Shape, grid: Grid, p: Parameters = Parameters(), *,
+                   precision="fp64", device=0):
+    """``transport_rate`` (core_shell.jl:62-70): (full n x n complex k-space
+    rate, FFTW convention; dissipation)."""
+    s = _ctx(shape, grid.spacing, p, precision, device)
+    c = _fortran(c)
+    re = np.empty_like(c, order="F")
+    im = np.empty_like(c, order="F")
+    d = np.zeros(1)
+    check(lib().csg_transport_rate(s.handle, ptr(c), ptr(re), ptr(im), ptr(d)))
+    return re + 1j * im, float(d[0])
+
+
+def energy(c, shape: Shape, grid: Grid, p: Parameters = Parameters(), *, precision="fp64", device=0):
+    """``energy`` (core_shell.jl:78-82)."""
+    s = _ctx(shape, grid.spacing, p, precision, device)
+    c = _fortran(c)
+    F = np.zeros(1)
+    check(lib().csg_energy(s.handle, ptr(c), ptr(F)))
+    return float(F[0])
+
+
+def step(c, shape: Shape, grid: Grid, p: Parameters, dt: float, *, precision="fp64", device=0):
+    """``step`` (core_shell.jl:72-76)."""
+    s = _session(shape.phi.shape[0], grid.spacing, p, precision, device)
+    s.set_fields(c, shape.phi)
+    s.step(dt, 1)
+    return s.field()
+
+
+def run_simulation(c0, phi, dt: float = 0.01, steps: int = 1000, spacing: float = 1.0,
+                   p: Parameters = Parameters(), directory=None, *, precision: str = "fp64",
+                   history_every: int = 1, device: int = 0):
+    """``run_simulation`` (core_shell.jl:84-103): returns (c, history) with
+    history rows [t, mean c, F, min c, max c, dissipation] of c_s for
+    s = 0, history_every, ..., before c_s is advanced.  c0 is not modified.
+    Raises NonFiniteCompositionError (and writes no files) on non-finite c_s."""
+    c0 = _fortran(c0)
+    s = _session(c0.shape[0], spacing, p, precision, device)
+    s.set_fields(c0, phi)
+    hist = s.run(dt, steps, history_every)
+    c = s.field()
+    if directory is not None:
+        from .writers import writedlm
+        os.makedirs(directory, exist_ok=True)
+        writedlm(os.path.join(directory, "history.csv"), hist)
+        writedlm(os.path.join(directory, "composition.csv"), c)
+        writedlm(os.path.join(directory, "particle.csv"), _fortran(phi))
+    return c, hist

@@ -1,0 +1,672 @@
+// C ABI of the core-shell step library: context, device buffers, per-size
+// kernel dispatch, the graph-captured time loop (core_shell.jl:84-103) and the
+// single-evaluation operators (energy, chemical potential, transport rate).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/csg.h"
+#include "csg_kernels.hpp"
+
+using namespace csg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(CSG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
+
+// Kernel table for one (N, precision).
+struct Ops {
+  LaunchFn fwd;        // real rows (double) -> state S (line layout, double)
+  LaunchFn inv_state;  // state S -> real rows (double)
+  LaunchFn inv_bulk;   // bulk-precision line array -> real rows (double)
+  LaunchFn p1, p2, p3, p4, mu, rate;
+  int n_row1, n_col, n_row3;  // workgroups of P1, P2, P3 (partials count)
+};
+
+template <class CT, class TIN, class TOUT, int N, int MODE>
+void launch_row(const PassArgs& a, const void* tw, hipStream_t s) {
+  using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
+  hipLaunchKernelGGL((row_pass<CT, TIN, TOUT, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
+                     static_cast<const cplx<CT>*>(tw));
+}
+
+template <class CT, class TS, int N, int MODE>
+void launch_col(const PassArgs& a, const void* tw, hipStream_t s) {
+  using Cfg = ColCfg<CT, TS, N, MODE>;
+  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
+                     static_cast<const cplx<CT>*>(tw));
+}
+
+// T: bulk (compute) type; the state is always double.
+template <int N, class T>
+Ops make_ops() {
+  Ops o;
+  o.fwd = launch_row<double, double, double, N, ROW_FWD>;
+  o.inv_state = launch_row<double, double, double, N, ROW_INV>;
+  o.inv_bulk = launch_row<T, T, double, N, ROW_INV>;
+  o.p1 = launch_row<T, double, T, N, ROW_P1>;
+  o.p3 = launch_row<T, T, T, N, ROW_P3>;
+  o.p2 = launch_col<T, double, N, COL_P2>;
+  o.mu = launch_col<T, double, N, COL_MU>;
+  o.p4 = launch_col<T, double, N, COL_P4>;
+  o.rate = launch_col<T, double, N, COL_RATE>;
+  o.n_row1 = RowCfg<T, double, T, N, ROW_P1>::GRID;
+  o.n_row3 = RowCfg<T, T, T, N, ROW_P3>::GRID;
+  o.n_col = ColCfg<T, double, N, COL_P2>::GRID;
+  return o;
+}
+
+template <class T>
+bool ops_for(int n, Ops* o) {
+  switch (n) {
+    case 16: *o = make_ops<16, T>(); return true;
+    case 32: *o = make_ops<32, T>(); return true;
+    case 64: *o = make_ops<64, T>(); return true;
+    case 128: *o = make_ops<128, T>(); return true;
+    case 256: *o = make_ops<256, T>(); return true;
+    case 512: *o = make_ops<512, T>(); return true;
+    case 1024: *o = make_ops<1024, T>(); return true;
+    case 2048: *o = make_ops<2048, T>(); return true;
+    case 4096: *o = make_ops<4096, T>(); return true;
+    default: return false;
+  }
+}
+
+constexpr int FIN_NT = 256;
+constexpr int STEPS_PER_GRAPH = 32;
+
+}  // namespace
+
+struct csg_ctx {
+  int n = 0;
+  int precision = 0;
+  int device = 0;
+  double spacing = 1.0;
+  csg_params p{};
+  DevParams dp{};
+  Ops ops{};
+  hipStream_t stream = nullptr;
+  size_t tsize = 8;       // bytes per bulk real
+  void* S = nullptr;      // state (double, line layout)
+  void* S2 = nullptr;     // scratch state for single evaluations
+  void* A = nullptr;      // F' / Gx / Jx (bulk)
+  void* Bb = nullptr;     // Gy / Jy (bulk)
+  void* phi = nullptr;    // real rows (bulk)
+  double* real = nullptr; // host I/O staging, N*N doubles
+  void* tw = nullptr;     // twiddles (bulk)
+  void* tw64 = nullptr;   // twiddles (double) for state transforms
+  double* part = nullptr; // partials: P1 (5 each), P2, P3
+  long long* ctr = nullptr;
+  double* hist = nullptr;
+  size_t hist_cap = 0;    // rows
+  double* scratch_row = nullptr;
+  double* rate = nullptr; // (N/2+1) x N complex doubles
+  bool fields_set = false;
+  long long nonfinite = -1;
+  hipGraphExec_t gexec = nullptr;
+  double gdt = 0.0;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int dmalloc(csg_ctx* c, void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  c->bytes += bytes;
+  return CSG_OK;
+}
+
+PassArgs base_args(const csg_ctx* c) {
+  PassArgs a{};
+  a.p = c->dp;
+  a.phi = c->phi;
+  return a;
+}
+
+// Partials sub-buffers.
+double* part1(const csg_ctx* c) { return c->part; }
+double* part2(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1; }
+double* part3(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1 + c->ops.n_col; }
+
+FinalArgs final_args(const csg_ctx* c, double* hist, double dt, long long step0) {
+  FinalArgs f{};
+  f.p1 = part1(c);
+  f.p2 = part2(c);
+  f.p3 = part3(c);
+  f.n1 = c->ops.n_row1;
+  f.n2 = c->ops.n_col;
+  f.n3 = c->ops.n_row3;
+  f.hist = hist;
+  f.ctr = c->ctr;
+  f.dt = dt;
+  f.spacing2 = c->spacing * c->spacing;
+  f.inv_n2 = 1.0 / (static_cast<double>(c->n) * c->n);
+  f.kappa = c->p.kappa;
+  f.step0 = step0;
+  return f;
+}
+
+// P1..P3 on state `S`: partials + Gx/Gy (A/Bb) then Jx/Jy (A/Bb).
+void enqueue_p123(csg_ctx* c, const void* S) {
+  PassArgs a = base_args(c);
+  a.in0 = S;
+  a.out0 = c->A;
+  a.part = part1(c);
+  c->ops.p1(a, c->tw, c->stream);
+  a = base_args(c);
+  a.in0 = S;
+  a.in1 = c->A;
+  a.out0 = c->A;
+  a.out1 = c->Bb;
+  a.part = part2(c);
+  c->ops.p2(a, c->tw, c->stream);
+  a = base_args(c);
+  a.in0 = c->A;
+  a.in1 = c->Bb;
+  a.out0 = c->A;
+  a.out1 = c->Bb;
+  a.part = part3(c);
+  c->ops.p3(a, c->tw, c->stream);
+}
+
+void enqueue_p4(csg_ctx* c, void* S, double dt) {
+  PassArgs a = base_args(c);
+  a.in0 = c->A;
+  a.in1 = c->Bb;
+  a.out0 = S;
+  a.dt = dt;
+  c->ops.p4(a, c->tw, c->stream);
+}
+
+void enqueue_finalize(csg_ctx* c, double* hist, double dt, long long step0) {
+  hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(1), dim3(FIN_NT), 0, c->stream,
+                     final_args(c, hist, dt, step0));
+}
+
+// One loop iteration of run_simulation with history.
+void enqueue_iteration(csg_ctx* c, double dt, bool advance) {
+  enqueue_p123(c, c->S);
+  enqueue_finalize(c, c->hist, dt, 0);
+  if (advance) enqueue_p4(c, c->S, dt);
+}
+
+int upload_real(csg_ctx* c, const double* host) {
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  HIPCHK(hipMemcpyAsync(c->real, host, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return CSG_OK;
+}
+
+int set_state_from_real(csg_ctx* c, void* S) {
+  PassArgs a = base_args(c);
+  a.in0 = c->real;
+  a.out0 = S;
+  c->ops.fwd(a, c->tw64, c->stream);
+  HIPCHK(hipGetLastError());
+  return CSG_OK;
+}
+
+int ensure_hist(csg_ctx* c, size_t rows) {
+  if (rows <= c->hist_cap) return CSG_OK;
+  if (c->gexec) {  // the captured graph holds the old history pointer
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  if (c->hist) {
+    HIPCHK(hipFree(c->hist));
+    c->bytes -= c->hist_cap * 6 * sizeof(double);
+  }
+  c->hist = nullptr;
+  c->hist_cap = 0;
+  int rc = dmalloc(c, reinterpret_cast<void**>(&c->hist), rows * 6 * sizeof(double));
+  if (rc) return rc;
+  c->hist_cap = rows;
+  return CSG_OK;
+}
+
+int reset_ctr(csg_ctx* c) {
+  const long long init[2] = {0, -1};
+  HIPCHK(hipMemcpyAsync(c->ctr, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  return CSG_OK;
+}
+
+int build_graph(csg_ctx* c, double dt) {
+  if (c->gexec && c->gdt == dt) return CSG_OK;
+  if (c->gexec) {
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_iteration(c, dt, true);
+  HIPCHK(hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  c->gdt = dt;
+  return CSG_OK;
+}
+
+int check_ctx(const csg_ctx* c) {
+  if (!c) return fail(CSG_ERR_ARG, "null context");
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void csg_default_params(csg_params* p) {
+  if (!p) return;
+  p->A = 2.0;
+  p->B = 4.0;
+  p->chi = 0.5;
+  p->P = 5.0;
+  p->ca = 0.5;
+  p->cb = 0.0;
+  p->cc = 1.0;
+  p->kappa = 2.0;
+  p->mobility = 1.0;
+  p->Q = 0.5;
+}
+
+const char* csg_last_error(void) { return g_err.c_str(); }
+
+int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, int32_t precision,
+               int32_t device) {
+  if (!out) return fail(CSG_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!(spacing > 0.0) || !std::isfinite(spacing)) return fail(CSG_ERR_ARG, "spacing must be > 0");
+  if (precision != CSG_FP64 && precision != CSG_MIXED) return fail(CSG_ERR_ARG, "unknown precision");
+  csg_ctx* c = new csg_ctx();
+  c->n = n;
+  c->precision = precision;
+  c->device = device;
+  c->spacing = spacing;
+  if (p) c->p = *p; else csg_default_params(&c->p);
+  const bool ok = (precision == CSG_FP64) ? ops_for<double>(n, &c->ops) : ops_for<float>(n, &c->ops);
+  if (!ok) {
+    delete c;
+    return fail(CSG_ERR_UNSUPPORTED, "grid size " + std::to_string(n) +
+                                         " not supported (N = 2^m, 16 <= N <= 4096)");
+  }
+  c->dp = DevParams{c->p.A, c->p.B, c->p.chi, c->p.P, c->p.ca, c->p.cb, c->p.cc,
+                    c->p.kappa, c->p.mobility, c->p.Q, spacing, (1.0 / spacing) / n};
+  c->tsize = (precision == CSG_FP64) ? sizeof(double) : sizeof(float);
+  const size_t nn = static_cast<size_t>(n) * n;
+  int rc;
+  auto cleanup = [&](int code) { csg_destroy(c); return code; };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+  if ((rc = dmalloc(c, &c->S, nn * sizeof(double)))) return cleanup(rc);
+  if ((rc = dmalloc(c, &c->A, nn * c->tsize))) return cleanup(rc);
+  if ((rc = dmalloc(c, &c->Bb, nn * c->tsize))) return cleanup(rc);
+  if ((rc = dmalloc(c, &c->phi, nn * c->tsize))) return cleanup(rc);
+  if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->real), nn * sizeof(double)))) return cleanup(rc);
+  const size_t npart = 5 * c->ops.n_row1 + c->ops.n_col + c->ops.n_row3;
+  if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->part), npart * sizeof(double)))) return cleanup(rc);
+  if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->ctr), 2 * sizeof(long long)))) return cleanup(rc);
+  if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->scratch_row), 8 * sizeof(double)))) return cleanup(rc);
+  // twiddles exp(-2 pi i m / N), extended precision, rounded once
+  std::vector<double> tw64(2 * n);
+  for (int m = 0; m < n; ++m) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
+    tw64[2 * m] = static_cast<double>(cosl(ang));
+    tw64[2 * m + 1] = static_cast<double>(sinl(ang));
+  }
+  if ((rc = dmalloc(c, &c->tw64, 2 * n * sizeof(double)))) return cleanup(rc);
+  e = hipMemcpy(c->tw64, tw64.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+  if (precision == CSG_FP64) {
+    c->tw = c->tw64;
+  } else {
+    std::vector<float> tw32(2 * n);
+    for (int m = 0; m < n; ++m) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
+      tw32[2 * m] = static_cast<float>(cosl(ang));
+      tw32[2 * m + 1] = static_cast<float>(sinl(ang));
+    }
+    if ((rc = dmalloc(c, &c->tw, 2 * n * sizeof(float)))) return cleanup(rc);
+    e = hipMemcpy(c->tw, tw32.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+  }
+  *out = c;
+  return CSG_OK;
+}
+
+int csg_destroy(csg_ctx* c) {
+  if (!c) return CSG_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  void* bufs[] = {c->S, c->S2, c->A, c->Bb, c->phi, c->real, c->part, c->ctr, c->hist,
+                  c->scratch_row, c->rate, c->tw64};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CSG_OK;
+}
+
+int64_t csg_device_bytes(const csg_ctx* c) { return c ? static_cast<int64_t>(c->bytes) : 0; }
+
+int csg_set_fields(csg_ctx* c, const double* c0, const double* phi) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c0 || !phi) return fail(CSG_ERR_ARG, "c0 and phi must be non-NULL");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  if (c->precision == CSG_FP64) {
+    HIPCHK(hipMemcpyAsync(c->phi, phi, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> tmp(nn);
+    for (size_t i = 0; i < nn; ++i) tmp[i] = static_cast<float>(phi[i]);
+    HIPCHK(hipMemcpyAsync(c->phi, tmp.data(), nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (int rc = upload_real(c, c0)) return rc;
+  if (int rc = set_state_from_real(c, c->S)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->fields_set = true;
+  return CSG_OK;
+}
+
+int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* history) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (steps < 0 || hist_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/hist_every/dt");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_hist(c, static_cast<size_t>(steps) + 1)) return rc;
+  if (int rc = reset_ctr(c)) return rc;
+  int64_t done = 0;
+  if (steps >= 2 * STEPS_PER_GRAPH) {
+    if (int rc = build_graph(c, dt)) return rc;
+    while (steps - done >= STEPS_PER_GRAPH) {
+      HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+      done += STEPS_PER_GRAPH;
+    }
+  }
+  for (; done < steps; ++done) enqueue_iteration(c, dt, true);
+  enqueue_iteration(c, dt, false);
+  HIPCHK(hipGetLastError());
+  long long ctr[2];
+  HIPCHK(hipMemcpyAsync(ctr, c->ctr, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
+  const int64_t rows = steps / hist_every + 1;
+  std::vector<double> h;
+  if (history) {
+    h.resize(static_cast<size_t>(steps + 1) * 6);
+    HIPCHK(hipMemcpyAsync(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (history) {
+    for (int64_t r = 0; r < rows; ++r)
+      for (int col = 0; col < 6; ++col) history[r + col * rows] = h[static_cast<size_t>(r * hist_every) * 6 + col];
+  }
+  c->nonfinite = ctr[1];
+  if (ctr[1] >= 0)
+    return fail(CSG_ERR_NONFINITE, "Non-finite composition at step " + std::to_string(ctr[1]));
+  return CSG_OK;
+}
+
+int csg_step(csg_ctx* c, double dt, int64_t nsteps) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (nsteps < 0 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad nsteps/dt");
+  HIPCHK(hipSetDevice(c->device));
+  for (int64_t s = 0; s < nsteps; ++s) {
+    enqueue_p123(c, c->S);
+    enqueue_p4(c, c->S, dt);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CSG_OK;
+}
+
+int csg_get_field(csg_ctx* c, double* out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (!out) return fail(CSG_ERR_ARG, "out is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  PassArgs a = base_args(c);
+  a.in0 = c->S;
+  a.out0 = c->real;
+  c->ops.inv_state(a, c->tw64, c->stream);
+  HIPCHK(hipGetLastError());
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  HIPCHK(hipMemcpyAsync(out, c->real, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CSG_OK;
+}
+
+int64_t csg_nonfinite_step(const csg_ctx* c) { return c ? c->nonfinite : -1; }
+
+}  // extern "C"
+
+namespace {
+
+int prepare_single(csg_ctx* c, const double* cin) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!cin) return fail(CSG_ERR_ARG, "c is NULL");
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "phi not set (csg_set_fields)");
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->S2) {
+    if (int rc = dmalloc(c, &c->S2, static_cast<size_t>(c->n) * c->n * sizeof(double))) return rc;
+  }
+  if (int rc = upload_real(c, cin)) return rc;
+  return set_state_from_real(c, c->S2);
+}
+
+// History-style reduction of P1..P3 on S2 into scratch_row.
+int single_reduce(csg_ctx* c, double out[6]) {
+  enqueue_p123(c, c->S2);
+  if (int rc = reset_ctr(c)) return rc;
+  enqueue_finalize(c, c->scratch_row, 0.0, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->scratch_row, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_energy(csg_ctx* c, const double* cin, double* F) {
+  if (int rc = prepare_single(c, cin)) return rc;
+  double row[6];
+  if (int rc = single_reduce(c, row)) return rc;
+  if (F) *F = row[2];
+  return CSG_OK;
+}
+
+int csg_chemical_potential(csg_ctx* c, const double* cin, double* mu) {
+  if (int rc = prepare_single(c, cin)) return rc;
+  if (!mu) return fail(CSG_ERR_ARG, "mu is NULL");
+  PassArgs a = base_args(c);
+  a.in0 = c->S2;
+  a.out0 = c->A;
+  a.part = part1(c);
+  c->ops.p1(a, c->tw, c->stream);
+  a = base_args(c);
+  a.in0 = c->S2;
+  a.in1 = c->A;
+  a.out0 = c->Bb;
+  c->ops.mu(a, c->tw, c->stream);
+  a = base_args(c);
+  a.in0 = c->Bb;
+  a.out0 = c->real;
+  c->ops.inv_bulk(a, c->tw, c->stream);
+  HIPCHK(hipGetLastError());
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  HIPCHK(hipMemcpyAsync(mu, c->real, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CSG_OK;
+}
+
+int csg_transport_rate(csg_ctx* c, const double* cin, double* rate_re, double* rate_im,
+                       double* dissipation) {
+  if (int rc = prepare_single(c, cin)) return rc;
+  const int n = c->n;
+  const size_t half = static_cast<size_t>(n / 2 + 1) * n;
+  if (!c->rate) {
+    if (int rc = dmalloc(c, reinterpret_cast<void**>(&c->rate), half * 2 * sizeof(double))) return rc;
+  }
+  double row[6];
+  if (int rc = single_reduce(c, row)) return rc;   // leaves Jx/Jy in A/Bb
+  if (dissipation) *dissipation = row[5];
+  if (!rate_re && !rate_im) return CSG_OK;
+  PassArgs a = base_args(c);
+  a.in0 = c->A;
+  a.in1 = c->Bb;
+  a.out0 = c->rate;
+  c->ops.rate(a, c->tw, c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(half * 2);
+  HIPCHK(hipMemcpyAsync(h.data(), c->rate, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // h: line kx (0..N/2) of N values over ky.  Full spectrum (column-major,
+  // element (kx, ky) at kx + ky*N) by Hermitian symmetry for kx > N/2.
+  for (int ky = 0; ky < n; ++ky) {
+    for (int kx = 0; kx < n; ++kx) {
+      double re, im;
+      if (kx <= n / 2) {
+        re = h[2 * (static_cast<size_t>(kx) * n + ky)];
+        im = h[2 * (static_cast<size_t>(kx) * n + ky) + 1];
+      } else {
+        const int mx = n - kx, my = (n - ky) % n;
+        re = h[2 * (static_cast<size_t>(mx) * n + my)];
+        im = -h[2 * (static_cast<size_t>(mx) * n + my) + 1];
+      }
+      const size_t o = static_cast<size_t>(kx) + static_cast<size_t>(ky) * n;
+      if (rate_re) rate_re[o] = re;
+      if (rate_im) rate_im[o] = im;
+    }
+  }
+  return CSG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Pointwise single-call evaluations over the N*N real field.
+template <int MODE>
+int pointwise(csg_ctx* c, const double* cin, double* o0, double* o1, double* o2) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "phi not set (csg_set_fields)");
+  HIPCHK(hipSetDevice(c->device));
+  const long long nn = static_cast<long long>(c->n) * c->n;
+  const size_t bytes = static_cast<size_t>(nn) * sizeof(double);
+  double* outs[3] = {o0, o1, o2};
+  const int nout = (MODE == PT_GEOMETRY) ? 3 : 1;
+  // device staging: outputs in A/Bb/S2 regions sized >= N*N doubles
+  if (!c->S2) {
+    if (int rc = dmalloc(c, &c->S2, bytes)) return rc;
+  }
+  double* d_out[3] = {c->real, static_cast<double*>(c->S2), static_cast<double*>(c->S)};
+  void* d_tmp = nullptr;
+  if (MODE == PT_GEOMETRY) {
+    // S holds the resident state: use a temporary for the third output
+    HIPCHK(hipMalloc(&d_tmp, bytes));
+    d_out[2] = static_cast<double*>(d_tmp);
+  } else {
+    HIPCHK(hipMemcpyAsync(c->real, cin, bytes, hipMemcpyHostToDevice, c->stream));
+    d_out[0] = static_cast<double*>(c->S2);
+  }
+  const dim3 grid(static_cast<unsigned>((nn + 255) / 256));
+  if (c->precision == CSG_FP64)
+    hipLaunchKernelGGL((pointwise_kernel<double, MODE>), grid, dim3(256), 0, c->stream, c->real,
+                       static_cast<const double*>(c->phi), d_out[0], d_out[1], d_out[2], nn, c->dp);
+  else
+    hipLaunchKernelGGL((pointwise_kernel<float, MODE>), grid, dim3(256), 0, c->stream, c->real,
+                       static_cast<const float*>(c->phi), d_out[0], d_out[1], d_out[2], nn, c->dp);
+  hipError_t e = hipGetLastError();
+  for (int k = 0; k < nout && e == hipSuccess; ++k)
+    if (outs[k]) e = hipMemcpyAsync(outs[k], d_out[k], bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_tmp) (void)hipFree(d_tmp);
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("pointwise: ") + hipGetErrorString(e));
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_geometry(csg_ctx* c, double* h, double* g, double* M) {
+  return pointwise<PT_GEOMETRY>(c, nullptr, h, g, M);
+}
+
+int csg_bulk_energy(csg_ctx* c, const double* cin, double* f) {
+  if (!cin || !f) return fail(CSG_ERR_ARG, "c and f must be non-NULL");
+  return pointwise<PT_BULK_ENERGY>(c, cin, f, nullptr, nullptr);
+}
+
+int csg_bulk_derivative(csg_ctx* c, const double* cin, double* fp) {
+  if (!cin || !fp) return fail(CSG_ERR_ARG, "c and fp must be non-NULL");
+  return pointwise<PT_BULK_DERIVATIVE>(c, cin, fp, nullptr, nullptr);
+}
+
+int csg_time_passes(csg_ctx* c, double dt, int64_t nsteps, double* ms) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (nsteps < 1 || !ms) return fail(CSG_ERR_ARG, "nsteps >= 1 and ms required");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_hist(c, static_cast<size_t>(nsteps) + 1)) return rc;
+  if (int rc = reset_ctr(c)) return rc;
+  hipEvent_t ev[6];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int64_t s = 0; s < nsteps; ++s) {
+    PassArgs a = base_args(c);
+    HIPCHK(hipEventRecord(ev[0], c->stream));
+    a.in0 = c->S; a.out0 = c->A; a.part = part1(c);
+    c->ops.p1(a, c->tw, c->stream);
+    HIPCHK(hipEventRecord(ev[1], c->stream));
+    a = base_args(c);
+    a.in0 = c->S; a.in1 = c->A; a.out0 = c->A; a.out1 = c->Bb; a.part = part2(c);
+    c->ops.p2(a, c->tw, c->stream);
+    HIPCHK(hipEventRecord(ev[2], c->stream));
+    a = base_args(c);
+    a.in0 = c->A; a.in1 = c->Bb; a.out0 = c->A; a.out1 = c->Bb; a.part = part3(c);
+    c->ops.p3(a, c->tw, c->stream);
+    HIPCHK(hipEventRecord(ev[3], c->stream));
+    enqueue_p4(c, c->S, dt);
+    HIPCHK(hipEventRecord(ev[4], c->stream));
+    enqueue_finalize(c, c->hist, dt, 0);
+    HIPCHK(hipEventRecord(ev[5], c->stream));
+    HIPCHK(hipEventSynchronize(ev[5]));
+    for (int k = 0; k < 5; ++k) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 5; ++k) ms[k] = acc[k] / static_cast<double>(nsteps);
+  return CSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+/*
+ * csg.h -- C ABI of the MI355X (gfx950) core-shell Cahn-Hilliard step library.
+ *
+ * Replaces the time-step loop of core_shell.jl (run_simulation, :84-103) and
+ * the functions it calls (step :72-76, transport_rate :62-70,
+ * chemical_potential :57-60, energy :78-82, bulk_derivative :51-55,
+ * bulk_energy :45-49, geometry :39-43, make_grid :17-25) with HIP kernels on
+ * one GPU.  The reference exposes these as plain Julia functions; a Julia
+ * caller binds the entry points below with `ccall` (the host package shipped
+ * beside this library mirrors the same API in Python through ctypes).
+ *
+ * Conventions
+ *   - Every entry point returns a csg_status (0 = CSG_OK).  No C++ exception
+ *     crosses the ABI.  csg_last_error() returns a thread-local message.
+ *   - Real fields are N x N float64 in Julia column-major order: element
+ *     (i, j) (0-based, i = first index, along which k_x varies) at i + j*N.
+ *     Host buffers are copied; the caller keeps ownership.
+ *   - Calls are synchronous on return.  One context per host thread; a
+ *     context is not re-entrant.
+ *   - Supported grids: N = 2^m, 16 <= N <= 4096, spacing > 0.
+ */
+#ifndef CSG_H
+#define CSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum csg_status {
+  CSG_OK = 0,
+  CSG_ERR_ARG = 1,          /* invalid argument (size, pointer, precision) */
+  CSG_ERR_HIP = 2,          /* HIP runtime error (message in csg_last_error) */
+  CSG_ERR_NONFINITE = 3,    /* non-finite composition: see csg_nonfinite_step */
+  CSG_ERR_UNSUPPORTED = 4,  /* grid size not compiled in */
+  CSG_ERR_STATE = 5         /* fields not set */
+} csg_status;
+
+typedef enum csg_precision {
+  CSG_FP64 = 0,   /* FP64 throughout */
+  CSG_MIXED = 1   /* FP32 FFTs and pointwise work; FP64 state, k=0 mode, mean,
+                     reductions and stabilisation denominator */
+} csg_precision;
+
+/* Mirrors `Parameters` (core_shell.jl:4-15), field for field, same order. */
+typedef struct csg_params {
+  double A, B, chi, P, ca, cb, cc, kappa, mobility, Q;
+} csg_params;
+
+typedef struct csg_ctx csg_ctx;
+
+/* The defaults of core_shell.jl:5-14. */
+void csg_default_params(csg_params* p);
+
+/* Create a context for an N x N grid (make_grid, core_shell.jl:17-25).
+ * precision: csg_precision; device: HIP device ordinal. */
+int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p,
+               int32_t precision, int32_t device);
+int csg_destroy(csg_ctx* ctx);
+
+/* Upload c0 and phi (host, column-major N x N; copied).  geometry(phi, p)
+ * (core_shell.jl:39-43) is evaluated on the device inside the passes. */
+int csg_set_fields(csg_ctx* ctx, const double* c0, const double* phi);
+
+/* The loop of run_simulation (core_shell.jl:90-95): for s = 0..steps record
+ * the history row of c_s, stop on non-finite c_s, advance unless s == steps.
+ * history: caller-allocated, (rows x 6) column-major float64 with
+ * rows = steps / hist_every + 1 (rows s = 0, hist_every, 2*hist_every, ...);
+ * may be NULL.  Columns: [t, mean c, F, min c, max c, dissipation].
+ * Returns CSG_ERR_NONFINITE (state left at c_steps) if any c_s is
+ * non-finite; csg_nonfinite_step then gives the first such s. */
+int csg_run(csg_ctx* ctx, double dt, int64_t steps, int64_t hist_every, double* history);
+
+/* nsteps applications of step (core_shell.jl:72-76) to the resident field. */
+int csg_step(csg_ctx* ctx, double dt, int64_t nsteps);
+
+/* Download the resident composition field (column-major N x N). */
+int csg_get_field(csg_ctx* ctx, double* c_out);
+
+/* First step with non-finite composition in the last csg_run, or -1. */
+int64_t csg_nonfinite_step(const csg_ctx* ctx);
+
+/* chemical_potential(c, shape, grid, p) (core_shell.jl:57-60).  c, mu: host
+ * column-major N x N.  Does not touch the resident field. */
+int csg_chemical_potential(csg_ctx* ctx, const double* c, double* mu);
+
+/* transport_rate(c, shape, grid, p) (core_shell.jl:62-70): the full N x N
+ * k-space rate in FFTW convention (unnormalised forward transform), split into
+ * real and imaginary column-major arrays, and the dissipation scalar.  Any of
+ * the output pointers may be NULL. */
+int csg_transport_rate(csg_ctx* ctx, const double* c, double* rate_re, double* rate_im,
+                       double* dissipation);
+
+/* energy(c, shape, grid, p) (core_shell.jl:78-82). */
+int csg_energy(csg_ctx* ctx, const double* c, double* F);
+
+/* geometry(phi, p) (core_shell.jl:39-43) of the uploaded phi: h, g, M
+ * (column-major N x N; any may be NULL). */
+int csg_geometry(csg_ctx* ctx, double* h, double* g, double* M);
+
+/* bulk_energy(c, shape, p) (core_shell.jl:45-49), pointwise f(c, phi). */
+int csg_bulk_energy(csg_ctx* ctx, const double* c, double* f);
+
+/* bulk_derivative(c, shape, p) (core_shell.jl:51-55), pointwise f'(c, phi). */
+int csg_bulk_derivative(csg_ctx* ctx, const double* c, double* fp);
+
+/* initial_fields(n; spacing, radius, width, seed) (core_shell.jl:27-37),
+ * host only: phi = 0.5(1 - tanh((r - radius)/width)) about index n/2, and
+ * c = 0.5 + interior-mean-free noise 0.01(2 rand - 1) inside phi >= 0.5,
+ * with rand = Julia's rand(MersenneTwister(seed), n, n) bit for bit
+ * (Julia >= 1.11 seeding, dSFMT-19937).  Any n >= 1. */
+int csg_initial_fields(int32_t n, double spacing, double radius, double width, int64_t seed,
+                       double* c_out, double* phi_out);
+
+/* Julia's rand(MersenneTwister(seed), count) in column-major order, host only. */
+int csg_julia_rand(int64_t seed, int64_t count, double* out);
+
+/* Per-pass device time of one step, averaged over nsteps steps timed with HIP
+ * events on the context's stream: ms[0..4] = P1, P2, P3, P4, finalize. */
+int csg_time_passes(csg_ctx* ctx, double dt, int64_t nsteps, double* ms);
+
+/* Bytes of device memory held by the context. */
+int64_t csg_device_bytes(const csg_ctx* ctx);
+
+const char* csg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSG_H */

@@ -1,0 +1,115 @@
+"""Host-side tests (no GPU): the C ABI library loads and exports every symbol
+include/csg.h declares; the product's host RNG / initial_fields match the
+oracle bit for bit; argument validation; the Julia CSV number format."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import core_shell_oracle as o
+from oracle import julia_rng
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "csg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(csg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    L = pkg.lib()
+    decl = _declared_symbols()
+    assert len(decl) >= 19
+    for name in decl:
+        assert hasattr(L, name), name
+    from csg_amd import _capi
+    assert set(decl) == set(_capi.EXPORTS)
+
+
+def test_default_params_match_reference(pkg):
+    from csg_amd import _capi
+    p = _capi.CsgParams()
+    pkg.lib().csg_default_params(ctypes.byref(p))
+    ref = o.Parameters()
+    for f in ("A", "B", "chi", "P", "ca", "cb", "cc", "kappa", "mobility", "Q"):
+        assert getattr(p, f) == getattr(ref, f) == getattr(pkg.Parameters(), f)
+
+
+@pytest.mark.parametrize("seed,count", [(5749, 4096), (5749, 1000), (5749, 7), (0, 385), (1, 384),
+                                        (123456789012, 2049), (-3, 500)])
+def test_julia_rand_matches_oracle(pkg, seed, count):
+    out = np.empty(count)
+    rc = pkg.lib().csg_julia_rand(seed, count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    ref = julia_rng.MersenneTwister(seed).rand(count)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("n,radius,width,spacing", [(64, 16.0, 4.0, 1.0), (256, 70.0, 4.0, 1.0),
+                                                    (17, 5.0, 2.0, 0.5), (128, 38.4, 2.0, 1.0)])
+def test_initial_fields_match_oracle(pkg, n, radius, width, spacing):
+    c, phi = pkg.initial_fields(n, spacing=spacing, radius=radius, width=width)
+    cr, pr = o.initial_fields(n, spacing=spacing, radius=radius, width=width)
+    # libm tanh vs numpy tanh: last-ulp differences of values near 1, which
+    # the cancellation in 1 - tanh turns into absolute differences <= 1 ulp(1)
+    assert np.max(np.abs(phi - pr)) <= 2.3e-16
+    assert np.array_equal(phi >= 0.5, pr >= 0.5)
+    assert np.max(np.abs(c - cr)) <= 1e-17   # mean subtraction may differ in the last ulp
+
+
+def test_initial_fields_known_answer_energy_via_oracle(pkg):
+    """The product's initial field gives check-results.txt's F(t=0) under the oracle."""
+    c, phi = pkg.initial_fields(64, radius=16.0)
+    F0 = o.energy(np.asarray(c), o.geometry(np.asarray(phi), o.Parameters()), o.make_grid(64, 1.0),
+                  o.Parameters())
+    assert abs(F0 / 329.48858453024934 - 1) < 2e-15
+
+
+@pytest.mark.parametrize("n", [8, 48, 100, 8192])
+def test_unsupported_grid_rejected_without_gpu(pkg, n):
+    from csg_amd import _capi
+    h = _capi._VP()
+    rc = pkg.lib().csg_create(h, n, 1.0, None, 0, 0)
+    assert rc == _capi.CSG_ERR_UNSUPPORTED
+    assert "not supported" in pkg.lib().csg_last_error().decode()
+
+
+def test_bad_arguments_rejected_without_gpu(pkg):
+    from csg_amd import _capi
+    L = pkg.lib()
+    h = _capi._VP()
+    assert L.csg_create(h, 64, 0.0, None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_create(h, 64, 1.0, None, 7, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_create(None, 64, 1.0, None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_run(None, 0.01, 10, 1, None) == _capi.CSG_ERR_ARG
+    assert L.csg_nonfinite_step(None) == -1
+    assert L.csg_initial_fields(0, 1.0, 1.0, 1.0, 1, None, None) == _capi.CSG_ERR_ARG
+
+
+@pytest.mark.parametrize("x,s", [(0.5, "0.5"), (1.0, "1.0"), (1e-5, "1.0e-5"), (0.0001, "0.0001"),
+                                 (4406.5104552251505, "4406.5104552251505"), (-0.0, "-0.0"),
+                                 (0.0, "0.0"), (100000.0, "100000.0"), (1e6, "1.0e6"),
+                                 (1234567.0, "1.234567e6"), (123456.7, "123456.7"),
+                                 (0.007360143442235456, "0.007360143442235456"),
+                                 (-1.25e-7, "-1.25e-7"), (6.661338147750939e-16, "6.661338147750939e-16"),
+                                 (float("nan"), "NaN"), (float("inf"), "Inf"), (2.0 ** 70, "1.1805916207174113e21")])
+def test_julia_float_format(pkg, x, s):
+    assert pkg.julia_float(x) == s
+
+
+def test_writedlm_layout(pkg, tmp_path):
+    a = np.array([[0.5, 1.0], [1e-5, -2.5]], order="F")
+    p = tmp_path / "m.csv"
+    pkg.writedlm(str(p), a)
+    assert p.read_text() == "0.5,1.0\n1.0e-5,-2.5\n"   # row i = first index
+
+
+def test_make_grid_matches_oracle(pkg):
+    for n, sp in ((16, 1.0), (64, 0.5), (15, 2.0)):
+        g = pkg.make_grid(n, sp)
+        r = o.make_grid(n, sp)
+        assert np.array_equal(g.kx, r["kx"]) and np.array_equal(g.ky, r["ky"]) and np.array_equal(g.k2, r["k2"])
