@@ -66,8 +66,8 @@ def test_pointwise_matches_oracle(pkg, n):
     c, phi = _fields(pkg, n)
     shape = pkg.geometry(phi)
     ref = o.geometry(phi, P)
-    for k in ("h", "g", "M"):
-        assert np.max(np.abs(getattr(shape, k) - ref[k])) < 1e-15
+    for k in ("h", "g", "M"):   # values in [0, 1]; the device contracts to FMA
+        assert np.max(np.abs(getattr(shape, k) - ref[k])) < 1e-14
     assert np.max(np.abs(pkg.bulk_energy(c, shape) - o.bulk_energy(c, ref, P))) < 1e-14
     assert np.max(np.abs(pkg.bulk_derivative(c, shape) - o.bulk_derivative(c, ref, P))) < 1e-14
 
