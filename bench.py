@@ -1,0 +1,188 @@
+"""Benchmark: lattice-site updates/s of the core-shell time loop at 1024^2.
+
+Workload (one "step" = one iteration of run_simulation's loop, core_shell.jl:
+90-95, history row included): N = 1024, R = 320, w = 2, dt = 0.01, seed 5749,
+default Parameters, fields resident in HBM before the timed region.  The FP64
+form and the mixed form (FP32 FFTs/pointwise, FP64 state and reductions) run
+side by side on the same inputs; the mixed form is reported as `value` when
+its max|dc| against the FP64 GPU run stays <= 1e-5, else the FP64 form.
+
+Also reported:
+  roofline      dominant pass of the reported form: algorithmic bytes per
+                launch (per-site bytes x N^2) / its mean HIP-event duration
+  cpu_baseline  the FP64 numpy oracle (reference-equivalent: 17 complex
+                N^2 FFTs per iteration), 1 thread, on a bounded sample of the
+                same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per site and pass (P1..P4); see the pass table in the
+# kernel header: state S is always FP64, bulk arrays and phi are 8 or 4 bytes.
+PASS_BYTES = {
+    "fp64": (24.0, 32.0, 40.0, 32.0),
+    "mixed": (16.0, 20.0, 20.0, 24.0),
+}
+PASS_NAMES = ("P1_row", "P2_col", "P3_row", "P4_col")
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--radius", type=float, default=320.0)
+    ap.add_argument("--width", type=float, default=2.0)
+    ap.add_argument("--dt", type=float, default=0.01)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (3 times), the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    return ap.parse_args()
+
+
+def calibrate_copy(mib: int) -> None:
+    """hipMemcpy device-to-device of a known size (runtime copy kernel)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, n) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, n, 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": mib << 20, "repeats": 3}), flush=True)
+
+
+def _traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
+    if a summary for this kernel and workload has been committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def run_form(pkg, form, c0, phi, a):
+    s = pkg.Session(a.n, 1.0, pkg.Parameters(), form)
+    s.set_fields(c0, phi)
+    if a.warmup:
+        s.run(a.dt, a.warmup)                 # also captures the step graph
+    s.set_fields(c0, phi)
+    t0 = time.perf_counter()
+    hist = s.run(a.dt, a.steps)               # synchronous: returns after the stream drains
+    t1 = time.perf_counter()
+    c = s.field()
+    ms_pass = s.time_passes(a.dt, 50)
+    s.close()
+    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass)
+
+
+def cpu_baseline(c0, phi, a, c_gpu_steps=None):
+    from oracle import core_shell_oracle as orc
+    c0 = np.asarray(c0)
+    phi = np.asarray(phi)
+    t0 = time.perf_counter()
+    c, _ = orc.run_simulation(c0, phi, dt=a.dt, steps=a.cpu_steps)
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    # run_simulation evaluates steps+1 loop iterations (the last without advancing)
+    rate = a.n * a.n * (a.cpu_steps + 1) / secs
+    return c, dict(value=rate, unit="site-updates/s", cores=1, kind="port",
+                   sample=f"oracle run_simulation (numpy FP64, 17 complex {a.n}^2 FFTs per iteration), "
+                          f"{a.cpu_steps + 1} loop iterations of the {a.n}^2 workload in {secs:.1f} s")
+
+
+def main():
+    a = _args()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no "
+              f"multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    c0, phi = pkg.initial_fields(a.n, radius=a.radius, width=a.width, seed=5749)
+    forms = [f for f in a.forms.split(",") if f]
+    res = {f: run_form(pkg, f, c0, phi, a) for f in forms}
+    sites = a.n * a.n
+    workload = f"{a.n}x{a.n}"
+    out = {}
+    ref = res.get("fp64")
+    for f, r in res.items():
+        ups = sites * (a.steps + 1) / r["seconds"]
+        ms = r["ms_pass"]
+        per = [PASS_BYTES[f][k] * sites / (ms[k] * 1e-3) / 1e9 for k in range(4)]
+        dom = int(np.argmax(ms[:4]))
+        step_bytes = sum(PASS_BYTES[f]) * sites
+        o = dict(value=ups, ms_per_step=1e3 * r["seconds"] / (a.steps + 1),
+                 pass_ms=dict(zip(PASS_NAMES + ("finalize",), [float(x) for x in ms])),
+                 pass_gbs=dict(zip(PASS_NAMES, per)),
+                 step_gbs=step_bytes / (sum(ms) * 1e-3) / 1e9,
+                 dominant=PASS_NAMES[dom], dominant_gbs=per[dom], dominant_ms=float(ms[dom]),
+                 dominant_bytes=PASS_BYTES[f][dom] * sites,
+                 energy_final=float(r["hist"][-1, 2]),
+                 mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))))
+        if ref is not None and f != "fp64":
+            o["max_dc_vs_fp64"] = float(np.max(np.abs(r["c"] - ref["c"])))
+            o["rel_energy_err_vs_fp64"] = float(np.max(np.abs(r["hist"][:, 2] - ref["hist"][:, 2])
+                                                      / np.abs(ref["hist"][:, 2])))
+        out[f] = o
+    pick = "mixed" if ("mixed" in out and out["mixed"].get("max_dc_vs_fp64", 1.0) <= 1e-5) else forms[0]
+    head = out[pick]
+    dom = head["dominant"]
+    roof = dict(bound="hbm", achieved=head["dominant_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=head["dominant_gbs"] / HBM_PEAK_GBS, kernel=dom,
+                bytes_per_launch=head["dominant_bytes"], ms_per_launch=head["dominant_ms"],
+                traffic=_traffic(dom, f"{workload}_{pick}"))
+    line = {
+        "metric": "site-updates/s at 1024² (FP64 vs mixed, max|Δc|≤1e-5); % HBM roofline",
+        "value": head["value"], "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if pick == "fp64" else "f32+f64",
+        "data": "synthetic: initial_fields (Julia MersenneTwister(5749) noise), no checkpoint",
+        "config": {"workload": f"plain method {workload}, R={a.radius:g}, w={a.width:g}, "
+                               f"dt={a.dt:g}, {a.steps} steps, history every step",
+                   "form": pick, "grid": a.n},
+        "roofline": roof,
+        "forms": out,
+    }
+    if not a.no_cpu:
+        c_cpu, base = cpu_baseline(c0, phi, a)
+        # the same number of GPU steps for a parity figure at full size
+        s = pkg.Session(a.n, 1.0, pkg.Parameters(), "fp64")
+        s.set_fields(c0, phi)
+        s.run(a.dt, a.cpu_steps)
+        base["max_dc_gpu_fp64_vs_oracle"] = float(np.max(np.abs(s.field() - c_cpu)))
+        s.close()
+        line["cpu_baseline"] = base
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -146,7 +146,7 @@ double* part1(const csg_ctx* c) { return c->part; }
 double* part2(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1; }
 double* part3(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1 + c->ops.n_col; }
 
-FinalArgs final_args(const csg_ctx* c, double* hist, double dt, long long step0) {
+FinalArgs final_args(const csg_ctx* c, double* hist, double dt) {
   FinalArgs f{};
   f.p1 = part1(c);
   f.p2 = part2(c);
@@ -160,7 +160,6 @@ FinalArgs final_args(const csg_ctx* c, double* hist, double dt, long long step0)
   f.spacing2 = c->spacing * c->spacing;
   f.inv_n2 = 1.0 / (static_cast<double>(c->n) * c->n);
   f.kappa = c->p.kappa;
-  f.step0 = step0;
   return f;
 }
 
@@ -187,25 +186,28 @@ void enqueue_p123(csg_ctx* c, const void* S) {
   c->ops.p3(a, c->tw, c->stream);
 }
 
-void enqueue_p4(csg_ctx* c, void* S, double dt) {
+// P4 on state `S`; with `hist` non-NULL its workgroup 0 also writes the
+// history row of the step from the P1..P3 partials.
+void enqueue_p4(csg_ctx* c, void* S, double dt, double* hist) {
   PassArgs a = base_args(c);
   a.in0 = c->A;
   a.in1 = c->Bb;
   a.out0 = S;
   a.dt = dt;
+  a.fin = final_args(c, hist, dt);
   c->ops.p4(a, c->tw, c->stream);
 }
 
-void enqueue_finalize(csg_ctx* c, double* hist, double dt, long long step0) {
+void enqueue_finalize(csg_ctx* c, double* hist, double dt) {
   hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(1), dim3(FIN_NT), 0, c->stream,
-                     final_args(c, hist, dt, step0));
+                     final_args(c, hist, dt));
 }
 
-// One loop iteration of run_simulation with history.
+// One loop iteration of run_simulation with history (core_shell.jl:90-95).
 void enqueue_iteration(csg_ctx* c, double dt, bool advance) {
   enqueue_p123(c, c->S);
-  enqueue_finalize(c, c->hist, dt, 0);
-  if (advance) enqueue_p4(c, c->S, dt);
+  if (advance) enqueue_p4(c, c->S, dt, c->hist);
+  else enqueue_finalize(c, c->hist, dt);
 }
 
 int upload_real(csg_ctx* c, const double* host) {
@@ -437,7 +439,7 @@ int csg_step(csg_ctx* c, double dt, int64_t nsteps) {
   HIPCHK(hipSetDevice(c->device));
   for (int64_t s = 0; s < nsteps; ++s) {
     enqueue_p123(c, c->S);
-    enqueue_p4(c, c->S, dt);
+    enqueue_p4(c, c->S, dt, nullptr);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -482,7 +484,7 @@ int prepare_single(csg_ctx* c, const double* cin) {
 int single_reduce(csg_ctx* c, double out[6]) {
   enqueue_p123(c, c->S2);
   if (int rc = reset_ctr(c)) return rc;
-  enqueue_finalize(c, c->scratch_row, 0.0, 0);
+  enqueue_finalize(c, c->scratch_row, 0.0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->scratch_row, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -635,37 +637,57 @@ int csg_time_passes(csg_ctx* c, double dt, int64_t nsteps, double* ms) {
   if (nsteps < 1 || !ms) return fail(CSG_ERR_ARG, "nsteps >= 1 and ms required");
   HIPCHK(hipSetDevice(c->device));
   if (int rc = ensure_hist(c, static_cast<size_t>(nsteps) + 1)) return rc;
-  if (int rc = reset_ctr(c)) return rc;
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  // one real step first so A/Bb/partials hold the values of a live step
+  enqueue_p123(c, c->S);
+  enqueue_p4(c, c->S, dt, nullptr);
+  // the timed batches overwrite S/A/Bb repeatedly: keep copies to restore
+  void* keep = nullptr;
+  HIPCHK(hipMalloc(&keep, nn * (sizeof(double) + 2 * c->tsize)));
+  char* kp = static_cast<char*>(keep);
+  HIPCHK(hipMemcpyAsync(kp, c->S, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(kp + nn * sizeof(double), c->A, nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(kp + nn * (sizeof(double) + c->tsize), c->Bb, nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
   hipEvent_t ev[6];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  double acc[5] = {0, 0, 0, 0, 0};
+  if (int rc = reset_ctr(c)) return rc;
+  // each pass launched nsteps times back to back, as in the captured loop
+  HIPCHK(hipEventRecord(ev[0], c->stream));
   for (int64_t s = 0; s < nsteps; ++s) {
     PassArgs a = base_args(c);
-    HIPCHK(hipEventRecord(ev[0], c->stream));
     a.in0 = c->S; a.out0 = c->A; a.part = part1(c);
     c->ops.p1(a, c->tw, c->stream);
-    HIPCHK(hipEventRecord(ev[1], c->stream));
-    a = base_args(c);
+  }
+  HIPCHK(hipEventRecord(ev[1], c->stream));
+  for (int64_t s = 0; s < nsteps; ++s) {
+    PassArgs a = base_args(c);
     a.in0 = c->S; a.in1 = c->A; a.out0 = c->A; a.out1 = c->Bb; a.part = part2(c);
     c->ops.p2(a, c->tw, c->stream);
-    HIPCHK(hipEventRecord(ev[2], c->stream));
-    a = base_args(c);
+  }
+  HIPCHK(hipEventRecord(ev[2], c->stream));
+  for (int64_t s = 0; s < nsteps; ++s) {
+    PassArgs a = base_args(c);
     a.in0 = c->A; a.in1 = c->Bb; a.out0 = c->A; a.out1 = c->Bb; a.part = part3(c);
     c->ops.p3(a, c->tw, c->stream);
-    HIPCHK(hipEventRecord(ev[3], c->stream));
-    enqueue_p4(c, c->S, dt);
-    HIPCHK(hipEventRecord(ev[4], c->stream));
-    enqueue_finalize(c, c->hist, dt, 0);
-    HIPCHK(hipEventRecord(ev[5], c->stream));
-    HIPCHK(hipEventSynchronize(ev[5]));
-    for (int k = 0; k < 5; ++k) {
-      float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-      acc[k] += t;
-    }
+  }
+  HIPCHK(hipEventRecord(ev[3], c->stream));
+  for (int64_t s = 0; s < nsteps; ++s) enqueue_p4(c, c->S, dt, c->hist);
+  HIPCHK(hipEventRecord(ev[4], c->stream));
+  if (int rc = reset_ctr(c)) return rc;
+  for (int64_t s = 0; s < nsteps; ++s) enqueue_finalize(c, c->hist, dt);
+  HIPCHK(hipEventRecord(ev[5], c->stream));
+  HIPCHK(hipEventSynchronize(ev[5]));
+  for (int k = 0; k < 5; ++k) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+    ms[k] = t / static_cast<double>(nsteps);
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int k = 0; k < 5; ++k) ms[k] = acc[k] / static_cast<double>(nsteps);
+  HIPCHK(hipMemcpyAsync(c->S, kp, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->A, kp + nn * sizeof(double), nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Bb, kp + nn * (sizeof(double) + c->tsize), nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(keep));
   return CSG_OK;
 }
 

@@ -1,23 +1,32 @@
-// LDS-resident mixed-radix Stockham FFT engine for gfx950.
+// Register-resident mixed-radix Stockham FFT engine for gfx950.
 //
-// A workgroup owns NLINES independent lines of LEN complex values stored in
-// LDS with a line pitch of PITCH elements.  Each stage reads R values per
-// butterfly from LDS into registers, applies the stage twiddles and an
-// in-register radix-R DFT, and writes the results back in autosorted
-// (Stockham) order, so the output is in natural order and no bit reversal is
-// needed.  Stage s with radix R and stride product P (product of the radices
-// of the previous stages) maps butterfly i (0 <= i < LEN/R), k = i mod P, to
-//   x_r = in[i + r*LEN/R],  x_r *= W_{P*R}^{r*k},  y = DFT_R(x),
+// A line of LEN complex values is owned by TPL = LEN/E threads; thread t holds
+// the E elements t + TPL*j (j = 0..E-1) in registers v[j] ("canonical"
+// layout: lanes with consecutive t touch consecutive elements, so global
+// loads and stores of whole lines coalesce).  A plan of radices R_0..R_{S-1}
+// (each dividing E, product LEN) is applied as Stockham stages: stage s with
+// radix R and stride product P = R_0...R_{s-1} maps butterfly
+// i = t + TPL*m (m < E/R), k = i mod P, to
+//   x_r = in[i + r*LEN/R] = v[m + r*E/R],  x_r *= W_{PR}^{rk},  y = DFT_R(x),
 //   out[(i-k)*R + k + q*P] = y_q.
-// Radices are 8 while LEN allows, then 4 or 2; the workgroup is sized to
-// NLINES*LEN/8 threads so a radix-8 stage is one butterfly per thread.
+// Between stages the outputs go through a padded LDS line buffer and the next
+// stage reads its canonical inputs back; the last stage has P = LEN/R, k = i,
+// so its outputs land directly in canonical order in registers.  Inverse
+// transforms use the reversed plan, so a forward transform's outputs are
+// exactly the inputs of a following inverse transform (and vice versa):
+// pointwise work between them needs no data movement.
 //
-// Twiddles come from a table tw[m] = exp(-2*pi*i*m/NTAB) (host-computed in
-// extended precision and rounded once); W_LEN^e = tw[e * (NTAB/LEN)].
-// DIR = -1 is the forward (exp(-i...), unnormalised) transform, DIR = +1 the
-// inverse without the 1/LEN factor.
+// Twiddles W_NTAB^e come from a half table tw[m] = exp(-2*pi*i*m/NTAB),
+// 0 <= m < NTAB/2 (W^e = -tw[e - NTAB/2] above), rounded once from extended
+// precision on the host; a line of length LEN uses W_LEN^e = W_NTAB^{e*NTAB/LEN}.
+// DIR = -1: forward, unnormalised; DIR = +1: inverse without 1/LEN.
+//
+// The index helpers and in-register DFTs are host-callable so the plan and
+// stage indexing can be checked on the CPU.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#define CSG_HD __host__ __device__ __forceinline__
 
 namespace csg {
 
@@ -26,203 +35,291 @@ struct alignas(2 * sizeof(T)) cplx {
   T x, y;
 };
 
-template <class T> __device__ __forceinline__ cplx<T> mk(T x, T y) { return cplx<T>{x, y}; }
-template <class T> __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
-template <class T> __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
-template <class T> __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return {s * a.x, s * a.y}; }
-template <class T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+template <class T> CSG_HD cplx<T> mk(T x, T y) { return cplx<T>{x, y}; }
+template <class T> CSG_HD cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T> CSG_HD cplx<T> operator-(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T> CSG_HD cplx<T> operator*(T s, cplx<T> a) { return {s * a.x, s * a.y}; }
+template <class T> CSG_HD cplx<T> cmul(cplx<T> a, cplx<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
-template <class T> __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return {a.x, -a.y}; }
-// a * (i)  and  a * (-i)
-template <class T> __device__ __forceinline__ cplx<T> mul_i(cplx<T> a) { return {-a.y, a.x}; }
-template <class T> __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }
-// a * (DIR * i): the radix-4 rotation W_4^{-DIR}... i.e. W_4 = exp(DIR*i*pi/2)
-template <int DIR, class T> __device__ __forceinline__ cplx<T> rot(cplx<T> a) {
+template <class T> CSG_HD cplx<T> conj(cplx<T> a) { return {a.x, -a.y}; }
+template <class T> CSG_HD cplx<T> mul_i(cplx<T> a) { return {-a.y, a.x}; }
+template <class T> CSG_HD cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }
+// a * W_4 with W_4 = exp(DIR * i * pi / 2)
+template <int DIR, class T> CSG_HD cplx<T> rot(cplx<T> a) {
   if constexpr (DIR < 0) return mul_mi(a); else return mul_i(a);
 }
+template <class T> CSG_HD cplx<T> cvt(cplx<double> a) { return {static_cast<T>(a.x), static_cast<T>(a.y)}; }
+template <class T> CSG_HD cplx<T> cvt(cplx<float> a) { return {static_cast<T>(a.x), static_cast<T>(a.y)}; }
 
-template <class T> __device__ __forceinline__ cplx<T> cvt(cplx<double> a) {
-  return {static_cast<T>(a.x), static_cast<T>(a.y)};
-}
-template <class T> __device__ __forceinline__ cplx<T> cvt(cplx<float> a) {
-  return {static_cast<T>(a.x), static_cast<T>(a.y)};
-}
-
-// ---- in-register DFTs ------------------------------------------------------
-template <int DIR, class T> __device__ __forceinline__ void dft2(cplx<T>* x) {
-  cplx<T> a = x[0], b = x[1];
+// ---- in-register DFTs (x[r * STRIDE], r < R) --------------------------------
+template <int DIR, int ST, class T> CSG_HD void dft2(cplx<T>* x) {
+  const cplx<T> a = x[0], b = x[ST];
   x[0] = a + b;
-  x[1] = a - b;
+  x[ST] = a - b;
 }
 
-template <int DIR, class T> __device__ __forceinline__ void dft4(cplx<T>& x0, cplx<T>& x1, cplx<T>& x2, cplx<T>& x3) {
-  cplx<T> t0 = x0 + x2, t1 = x0 - x2, t2 = x1 + x3, t3 = rot<DIR>(x1 - x3);
+template <int DIR, class T> CSG_HD void dft4v(cplx<T>& x0, cplx<T>& x1, cplx<T>& x2, cplx<T>& x3) {
+  const cplx<T> t0 = x0 + x2, t1 = x0 - x2, t2 = x1 + x3, t3 = rot<DIR>(x1 - x3);
   x0 = t0 + t2;
   x1 = t1 + t3;
   x2 = t0 - t2;
   x3 = t1 - t3;
 }
 
-template <int DIR, class T> __device__ __forceinline__ void dft4(cplx<T>* x) { dft4<DIR>(x[0], x[1], x[2], x[3]); }
+template <int DIR, int ST, class T> CSG_HD void dft4(cplx<T>* x) { dft4v<DIR>(x[0], x[ST], x[2 * ST], x[3 * ST]); }
 
-template <int DIR, class T> __device__ __forceinline__ void dft8(cplx<T>* x) {
+// multiply by W_8^q (q = 1, 3) for direction DIR
+template <int DIR, class T> CSG_HD cplx<T> w8_1(cplx<T> o) {
   const T s = static_cast<T>(0.70710678118654752440084436210484903928);
-  cplx<T> e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
-  cplx<T> o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
-  dft4<DIR>(e0, e1, e2, e3);
-  dft4<DIR>(o0, o1, o2, o3);
-  // o_q *= W_8^q with W_8 = exp(DIR * i * pi / 4)
-  if constexpr (DIR < 0) {
-    o1 = cplx<T>{s * (o1.x + o1.y), s * (o1.y - o1.x)};
-    o3 = cplx<T>{s * (o3.y - o3.x), -s * (o3.x + o3.y)};
-  } else {
-    o1 = cplx<T>{s * (o1.x - o1.y), s * (o1.y + o1.x)};
-    o3 = cplx<T>{-s * (o3.x + o3.y), s * (o3.x - o3.y)};
-  }
+  if constexpr (DIR < 0) return {s * (o.x + o.y), s * (o.y - o.x)};
+  else return {s * (o.x - o.y), s * (o.y + o.x)};
+}
+template <int DIR, class T> CSG_HD cplx<T> w8_3(cplx<T> o) {
+  const T s = static_cast<T>(0.70710678118654752440084436210484903928);
+  if constexpr (DIR < 0) return {s * (o.y - o.x), -s * (o.x + o.y)};
+  else return {-s * (o.x + o.y), s * (o.x - o.y)};
+}
+
+template <int DIR, int ST, class T> CSG_HD void dft8(cplx<T>* x) {
+  cplx<T> e0 = x[0], e1 = x[2 * ST], e2 = x[4 * ST], e3 = x[6 * ST];
+  cplx<T> o0 = x[ST], o1 = x[3 * ST], o2 = x[5 * ST], o3 = x[7 * ST];
+  dft4v<DIR>(e0, e1, e2, e3);
+  dft4v<DIR>(o0, o1, o2, o3);
+  o1 = w8_1<DIR>(o1);
   o2 = rot<DIR>(o2);
-  x[0] = e0 + o0; x[4] = e0 - o0;
-  x[1] = e1 + o1; x[5] = e1 - o1;
-  x[2] = e2 + o2; x[6] = e2 - o2;
-  x[3] = e3 + o3; x[7] = e3 - o3;
+  o3 = w8_3<DIR>(o3);
+  x[0] = e0 + o0; x[4 * ST] = e0 - o0;
+  x[ST] = e1 + o1; x[5 * ST] = e1 - o1;
+  x[2 * ST] = e2 + o2; x[6 * ST] = e2 - o2;
+  x[3 * ST] = e3 + o3; x[7 * ST] = e3 - o3;
 }
 
-template <int R, int DIR, class T> __device__ __forceinline__ void dft(cplx<T>* x) {
-  if constexpr (R == 2) dft2<DIR>(x);
-  else if constexpr (R == 4) dft4<DIR>(x);
-  else dft8<DIR>(x);
+// radix 16 as 4 x 4: x[n1 + 4 n2] -> X[k2 + 4 k1]... computed in place with
+// the natural output order y[q] at x[q * ST].
+template <int DIR, int ST, class T> CSG_HD void dft16(cplx<T>* x) {
+  const T c1 = static_cast<T>(0.92387953251128675612818318939678828682);  // cos(pi/8)
+  const T s1 = static_cast<T>(0.38268343236508977172845998403039886676);  // sin(pi/8)
+  const T h = static_cast<T>(0.70710678118654752440084436210484903928);
+  cplx<T> a[16];
+#pragma unroll
+  for (int n = 0; n < 16; ++n) a[n] = x[n * ST];
+  // first pass: DFT4 over n2 for each n1 (elements n1, n1+4, n1+8, n1+12)
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) dft4v<DIR>(a[n1], a[n1 + 4], a[n1 + 8], a[n1 + 12]);
+  // twiddles W_16^{n1 * k2}: a[n1 + 4 k2] *= W16^(n1 k2)
+  // W16^1 = (c1, DIR*s1), W16^2 = (h, DIR*h), W16^3 = (s1, DIR*c1), W16^4 = (0, DIR), ...
+  const T d = static_cast<T>(DIR);
+  const cplx<T> w1{c1, d * s1}, w2{h, d * h}, w3{s1, d * c1}, w6{-h, d * h}, w9{-c1, -d * s1};
+  a[1 + 4] = cmul(a[1 + 4], w1);
+  a[1 + 8] = cmul(a[1 + 8], w2);
+  a[1 + 12] = cmul(a[1 + 12], w3);
+  a[2 + 4] = cmul(a[2 + 4], w2);
+  a[2 + 8] = rot<DIR>(a[2 + 8]);
+  a[2 + 12] = cmul(a[2 + 12], w6);
+  a[3 + 4] = cmul(a[3 + 4], w3);
+  a[3 + 8] = cmul(a[3 + 8], w6);
+  a[3 + 12] = cmul(a[3 + 12], w9);
+  // second pass: DFT4 over n1 for each k2 -> X[k2 + 4 k1]
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) dft4v<DIR>(a[4 * k2], a[4 * k2 + 1], a[4 * k2 + 2], a[4 * k2 + 3]);
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2)
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) x[(k2 + 4 * k1) * ST] = a[4 * k2 + k1];
 }
 
-template <int REM> constexpr int pick_radix() { return (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2); }
+template <int R, int DIR, int ST, class T> CSG_HD void dft(cplx<T>* x) {
+  if constexpr (R == 2) dft2<DIR, ST>(x);
+  else if constexpr (R == 4) dft4<DIR, ST>(x);
+  else if constexpr (R == 8) dft8<DIR, ST>(x);
+  else dft16<DIR, ST>(x);
+}
 
-// Twiddle W_{LEN}^{e} (direction DIR) from the table; TM = NTAB / LEN.
-template <int DIR, int TM, class T>
-__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* __restrict__ tw, int e) {
-  cplx<T> w = tw[e * TM];
+// ---- plans -----------------------------------------------------------------
+// Elements per thread: 16 for LEN in {16, 128, 256} and LEN >= 1024, else 8
+// (LEN <= 8: LEN).
+#ifndef CSG_FFT_E_BIG
+#define CSG_FFT_E_BIG 8
+#endif
+constexpr int fft_E(int len) {
+  return len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : (len >= 1024 ? CSG_FFT_E_BIG : 8));
+}
+// Radix of forward stage s: greedy largest radix <= E dividing what remains.
+constexpr int fft_radix(int len, int s) {
+  const int e = fft_E(len);
+  int rem = len;
+  int r = 1;
+  for (int k = 0; k <= s; ++k) {
+    r = 16;
+    while (r > e || rem % r != 0) r >>= 1;
+    rem /= r;
+  }
+  return r;
+}
+constexpr int fft_nstages(int len) {
+  int n = 0, rem = len;
+  while (rem > 1) {
+    rem /= fft_radix(len, n);
+    ++n;
+  }
+  return n;
+}
+// radix / stride product of stage s of the forward (INV = false) or inverse
+// (reversed) plan
+constexpr int plan_radix(int len, int s, bool inv) {
+  return inv ? fft_radix(len, fft_nstages(len) - 1 - s) : fft_radix(len, s);
+}
+constexpr int plan_pstride(int len, int s, bool inv) {
+  int p = 1;
+  for (int k = 0; k < s; ++k) p *= plan_radix(len, k, inv);
+  return p;
+}
+
+// LDS padding: one slot per 128 bytes of line data.
+template <class T> constexpr int pad_shift() { return sizeof(T) == 8 ? 3 : 4; }
+template <class T> CSG_HD int pad(int e) { return e + (e >> pad_shift<T>()); }
+template <class T> constexpr int padded_len(int len) { return len + (len >> pad_shift<T>()) + 1; }
+
+// Stockham output position of value q of butterfly i at stride product P.
+CSG_HD int stockham_out(int i, int q, int R, int P) {
+  const int k = i % P;
+  return (i - k) * R + k + q * P;
+}
+
+// ---- device side -------------------------------------------------------------
+
+// W_NTAB^e (0 <= e < NTAB) for direction DIR from the half table.
+template <int DIR, int NTAB, class T>
+CSG_HD cplx<T> tw_at(const cplx<T>* tw, int e) {
+  cplx<T> w = tw[e & (NTAB / 2 - 1)];
+  if (e & (NTAB / 2)) w = cplx<T>{-w.x, -w.y};
   if constexpr (DIR > 0) w.y = -w.y;
   return w;
 }
 
-// One Stockham stage over all lines resident in LDS.
-template <class T, int DIR, int LEN, int R, int P, int NLINES, int PITCH, int NT, int TM>
-__device__ __forceinline__ void stockham_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-  constexpr int NB = LEN / R;
-  constexpr int TOT = NB * NLINES;
-  constexpr int PER = (TOT + NT - 1) / NT;
-  cplx<T> v[PER][R];
-#pragma unroll
-  for (int t = 0; t < PER; ++t) {
-    const int f = threadIdx.x + t * NT;
-    if (TOT % NT == 0 || f < TOT) {
-      const int line = f / NB, i = f - line * NB;
-      const cplx<T>* L = buf + line * PITCH;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[t][r] = L[i + r * NB];
-      if constexpr (P > 1) {
-        const int k = i % P;
-        constexpr int STEP = LEN / (P * R);
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], twiddle<DIR, TM>(tw, r * k * STEP));
-      }
-      dft<R, DIR>(v[t]);
-    }
+template <int NTAB, int NT, class T>
+__device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __restrict__ g) {
+  for (int m = threadIdx.x; m < NTAB / 2; m += NT) lds_tw[m] = g[m];
+}
+
+// Order LDS traffic between stages: lines that fit in one wavefront only need
+// the wave's own accesses ordered; wider lines need the workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
   }
-  __syncthreads();
+}
+
+// Butterflies of one stage for one array, in registers.
+template <class T, int DIR, int LEN, int TPL, int E, int R, int P, int NTAB>
+CSG_HD void stage_compute(cplx<T> (&v)[E], int t, const cplx<T>* tw) {
+  constexpr int NBT = E / R;  // butterflies per thread
+  constexpr int TM = NTAB / LEN;
 #pragma unroll
-  for (int t = 0; t < PER; ++t) {
-    const int f = threadIdx.x + t * NT;
-    if (TOT % NT == 0 || f < TOT) {
-      const int line = f / NB, i = f - line * NB;
+  for (int m = 0; m < NBT; ++m) {
+    if constexpr (P > 1) {
+      const int i = t + TPL * m;
       const int k = i % P;
-      cplx<T>* L = buf + line * PITCH;
-      const int j = (i - k) * R + k;
+      constexpr int STEP = LEN / (P * R);
 #pragma unroll
-      for (int q = 0; q < R; ++q) L[j + q * P] = v[t][q];
+      for (int r = 1; r < R; ++r) v[m + r * NBT] = cmul(v[m + r * NBT], tw_at<DIR, NTAB>(tw, r * k * STEP * TM));
     }
+    dft<R, DIR, NBT>(&v[m]);
+  }
+}
+
+// Write a non-final stage's outputs to the line buffer, then read the next
+// stage's canonical inputs.
+template <class T, int LEN, int TPL, int E, int R, int P, bool WAVE>
+__device__ __forceinline__ void stage_exchange(cplx<T> (&v)[E], int t, cplx<T>* line) {
+  constexpr int NBT = E / R;
+#pragma unroll
+  for (int m = 0; m < NBT; ++m) {
+    const int i = t + TPL * m;
+#pragma unroll
+    for (int q = 0; q < R; ++q) line[pad<T>(stockham_out(i, q, R, P))] = v[m + q * NBT];
+  }
+  lds_sync<WAVE>();
+#pragma unroll
+  for (int j = 0; j < E; ++j) v[j] = line[pad<T>(t + TPL * j)];
+  if constexpr (!WAVE) __syncthreads();
+}
+
+template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE>
+__device__ __forceinline__ void fft_stages(cplx<T> (&v)[NA][fft_E(LEN)], int t, cplx<T>* line,
+                                           const cplx<T>* tw) {
+  constexpr bool INV = DIR > 0;
+  constexpr int NS = fft_nstages(LEN);
+  if constexpr (S < NS) {
+    constexpr int E = fft_E(LEN), TPL = LEN / E;
+    constexpr int R = plan_radix(LEN, S, INV), P = plan_pstride(LEN, S, INV);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB>(v[a], t, tw);
+    if constexpr (S + 1 < NS) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) stage_exchange<T, LEN, TPL, E, R, P, WAVE>(v[a], t, line);
+    }
+    fft_stages<T, DIR, LEN, NA, S + 1, NTAB, WAVE>(v, t, line, tw);
+  }
+}
+
+// Transform NA lines held in canonical registers by this thread group (TPL
+// threads, t = index within the group), using `line` (padded_len<T>(LEN)
+// elements of LDS private to the group) for the exchanges.
+template <class T, int DIR, int LEN, int NA, int NTAB, bool WAVE>
+__device__ __forceinline__ void fft_regs(cplx<T> (&v)[NA][fft_E(LEN)], int t, cplx<T>* line,
+                                         const cplx<T>* tw) {
+  fft_stages<T, DIR, LEN, NA, 0, NTAB, WAVE>(v, t, line, tw);
+}
+
+// ---- workgroup reductions (wave shuffles, one barrier) ---------------------
+enum RedOp { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+
+template <int OP> __device__ __forceinline__ double red_combine(double a, double b) {
+  if constexpr (OP == RED_SUM) return a + b;
+  // NaN-propagating min/max
+  else if constexpr (OP == RED_MIN) return (a != a || b != b) ? a + b : fmin(a, b);
+  else return (a != a || b != b) ? a + b : fmax(a, b);
+}
+
+template <int OP> __device__ __forceinline__ double wave_reduce(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = red_combine<OP>(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Reduce K values across the workgroup; every thread gets the results.
+// `scratch` must hold K * (NT/64) doubles.  Starts and ends with a barrier.
+template <int NT, int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], const int (&ops)[K], double* scratch) {
+  constexpr int NW = (NT + 63) / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (ops[q] == RED_SUM) v[q] = wave_reduce<RED_SUM>(v[q]);
+    else if (ops[q] == RED_MIN) v[q] = wave_reduce<RED_MIN>(v[q]);
+    else v[q] = wave_reduce<RED_MAX>(v[q]);
   }
   __syncthreads();
-}
-
-template <class T, int DIR, int LEN, int P, int NLINES, int PITCH, int NT, int TM>
-__device__ __forceinline__ void stockham_from(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-  if constexpr (P < LEN) {
-    constexpr int R = pick_radix<LEN / P>();
-    stockham_stage<T, DIR, LEN, R, P, NLINES, PITCH, NT, TM>(buf, tw);
-    stockham_from<T, DIR, LEN, P * R, NLINES, PITCH, NT, TM>(buf, tw);
-  }
-}
-
-// Full transform of every line. Requires a barrier before the call (data in
-// LDS complete); ends with a barrier.
-template <class T, int DIR, int LEN, int NLINES, int PITCH, int NT, int TM>
-__device__ __forceinline__ void fft_lines(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-  stockham_from<T, DIR, LEN, 1, NLINES, PITCH, NT, TM>(buf, tw);
-}
-
-// ---- real <-> packed half spectrum (rows of N = 2L reals) ------------------
-// Packed format of the R2C transform of a real row x[0..N): X[k] for
-// 1 <= k < L as usual, slot 0 holds (X[0], X[L]) as (re, im) -- both are real.
-
-// Forward post-processing: z = FFT_L(x as L complex) -> packed X (in place).
-// Each (k, L-k) pair is owned by one thread; requires the FFT output complete.
-template <class T, int N, int NLINES, int PITCH, int NT>
-__device__ __forceinline__ void r2c_post(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-  constexpr int L = N / 2;
-  constexpr int NP = L / 2 + 1;  // pairs k = 0 .. L/2
-  const T half = static_cast<T>(0.5);
-  for (int f = threadIdx.x; f < NLINES * NP; f += NT) {
-    const int line = f / NP, k = f - line * NP;
-    cplx<T>* X = buf + line * PITCH;
-    if (k == 0) {
-      const cplx<T> z = X[0];
-      X[0] = mk(z.x + z.y, z.x - z.y);
-    } else {
-      const int m = L - k;
-      const cplx<T> zk = X[k], zm = X[m];
-      // E_k = (Z_k + conj Z_m)/2, O_k = (Z_k - conj Z_m)/(2i)
-      const cplx<T> ek = half * (zk + conj(zm));
-      const cplx<T> ok = half * mul_mi(zk - conj(zm));
-      const cplx<T> wk = tw[k];  // W_N^k
-      const cplx<T> xk = ek + cmul(wk, ok);
-      if (m != k) {
-        const cplx<T> em = half * (zm + conj(zk));
-        const cplx<T> om = half * mul_mi(zm - conj(zk));
-        const cplx<T> wm = tw[m];
-        X[m] = em + cmul(wm, om);
-      }
-      X[k] = xk;
-    }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) scratch[q * NW + wave] = v[q];
   }
   __syncthreads();
-}
-
-// Inverse pre-processing: packed X -> Z such that x (as L complex) =
-// FFT^{-1}_L(Z) without normalisation; the overall 1/N is folded in via
-// `scale`.  In place, pairs owned by one thread.
-template <class T, int N, int NLINES, int PITCH, int NT>
-__device__ __forceinline__ void c2r_pre(cplx<T>* buf, const cplx<T>* __restrict__ tw, T scale) {
-  constexpr int L = N / 2;
-  constexpr int NP = L / 2 + 1;
-  for (int f = threadIdx.x; f < NLINES * NP; f += NT) {
-    const int line = f / NP, k = f - line * NP;
-    cplx<T>* X = buf + line * PITCH;
-    if (k == 0) {
-      const cplx<T> p = X[0];
-      X[0] = mk(scale * (p.x + p.y), scale * (p.x - p.y));
-    } else {
-      const int m = L - k;
-      const cplx<T> xk = X[k], xm = X[m];
-      // Z_k = (X_k + conj X_m) + i (X_k - conj X_m) W_N^{-k}
-      cplx<T> wk = tw[k];
-      wk.y = -wk.y;
-      const cplx<T> zk = scale * ((xk + conj(xm)) + mul_i(cmul(xk - conj(xm), wk)));
-      if (m != k) {
-        cplx<T> wm = tw[m];
-        wm.y = -wm.y;
-        X[m] = scale * ((xm + conj(xk)) + mul_i(cmul(xm - conj(xk), wm)));
-      }
-      X[k] = zk;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double a = scratch[q * NW];
+    for (int w = 1; w < NW; ++w) {
+      const double b = scratch[q * NW + w];
+      a = (ops[q] == RED_SUM) ? a + b : (ops[q] == RED_MIN ? red_combine<RED_MIN>(a, b) : red_combine<RED_MAX>(a, b));
     }
+    v[q] = a;
   }
   __syncthreads();
 }

@@ -13,9 +13,12 @@
 //   P1 row pass    S -> c (C2R along i) -> f'(c), bulk f, min/max -> F' (R2C)
 //   P2 column pass S, F' -> FFT_j -> mu^ = F'^ + 2 kappa K^2 c^ -> Gx, Gy
 //                  = i kd mu^ -> IFFT_j; gradient-energy partial
-//   P3 row pass    Gx, Gy -> gx, gy -> J = M0 h(phi) g, dissipation -> R2C
+//   P3 row pass    Gx or Gy -> g -> J = M0 h(phi) g, dissipation -> R2C
+//                  (one gradient component per workgroup)
 //   P4 column pass Jx, Jy -> FFT_j -> inc = dt (i kdx Jx^ + i kdy Jy^)/den
-//                  -> IFFT_j -> S += inc   (core_shell.jl:74-75, linear in c^)
+//                  -> IFFT_j -> S += inc   (core_shell.jl:74-75, linear in c^);
+//                  workgroup 0 also reduces the step's partials into the
+//                  history row.
 // Row passes read and write their tiles transposed (line layout), so column
 // passes see contiguous lines.
 #pragma once
@@ -46,6 +49,21 @@ __device__ __forceinline__ double dkval(int m, double inv_nsp) {
 enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_P1 = 2, ROW_P3 = 3 };
 enum ColMode { COL_P2 = 0, COL_MU = 1, COL_P4 = 2, COL_RATE = 3 };
 
+// Reduction of the per-workgroup partials into one history row
+// [t, mean c, F, min c, max c, dissipation] (core_shell.jl:92).
+struct FinalArgs {
+  const double* p1;   // 5 per P1 workgroup: sum f, sum of row sums, min, max, #non-finite
+  const double* p2;   // 1 per P2 workgroup: gradient-energy sum
+  const double* p3;   // 1 per P3 workgroup: dissipation sum
+  int n1, n2, n3;
+  double* hist;       // history buffer (6 per row); NULL: no history
+  long long* ctr;     // [0] = row index of this step, [1] = first non-finite step (-1)
+  double dt;
+  double spacing2;
+  double inv_n2;      // 1 / N^2
+  double kappa;
+};
+
 struct PassArgs {
   const void* in0;
   const void* in1;
@@ -55,146 +73,197 @@ struct PassArgs {
   double* part;      // per-workgroup partial records
   double dt;
   DevParams p;
+  FinalArgs fin;     // P4 only
 };
-
-// Workgroup shapes.  Row passes: B rows of L complex per array; column passes:
-// C lines of N complex per array.  Threads = lines*len/8 (one radix-8
-// butterfly per thread per stage).
-template <int N, int ELEM, int NARR>
-constexpr int row_B() {
-  // LDS budget per workgroup ~64 KiB, at least 1 and at most N rows.
-  int b = (64 * 1024) / (NARR * (N / 2 + 1) * 2 * ELEM);
-  int p = 1;
-  while (p * 2 <= b && p * 2 <= 8) p *= 2;
-  if (p > N) p = N;
-  // at least 64 threads where the grid allows it
-  while (NARR * p * (N / 2) / 8 < 64 && p * 2 <= N) p *= 2;
-  return p;
-}
-template <int N, int ELEM>
-constexpr int col_C() {
-  int c = (64 * 1024) / (2 * (N + 1) * 2 * ELEM);
-  int p = 1;
-  while (p * 2 <= c && p * 2 <= 8 && 2 * p * 2 * N / 8 <= 1024) p *= 2;
-  if (p > N / 2) p = N / 2;
-  while (2 * p * N / 8 < 64 && p * 2 <= N / 2) p *= 2;
-  return p;
-}
 
 template <class T> __device__ __forceinline__ bool finite(T v) { return isfinite(v); }
 
-// Block reduction helpers (deterministic tree over an LDS scratch of NT doubles).
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-#pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  double r = red[0];
-  __syncthreads();
-  return r;
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap so that each XCD receives a contiguous range of tiles: row
+// tiles j0 and j0+B share 128-byte lines of the line-layout arrays, which
+// are then fetched once into that XCD's L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int grid) {
+  if (grid % NUM_XCD != 0) return b;
+  return (b % NUM_XCD) * (grid / NUM_XCD) + b / NUM_XCD;
 }
-template <int NT, bool MAX>
-__device__ __forceinline__ double block_minmax(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-#pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      double a = red[threadIdx.x], b = red[threadIdx.x + s];
-      // NaN-propagating: a NaN anywhere poisons the result
-      red[threadIdx.x] = (a != a || b != b) ? (a + b) : (MAX ? fmax(a, b) : fmin(a, b));
-    }
-    __syncthreads();
+
+// The twiddle half table (N/2 entries) is staged in LDS when it takes at most
+// 16 KiB; larger tables (N = 4096 in FP64) are read through the caches.
+template <int N, class CT>
+constexpr bool tw_in_lds() { return (N / 2) * 2 * sizeof(CT) <= 16384; }
+template <int N, class CT>
+constexpr int tw_lds_len() { return tw_in_lds<N, CT>() ? N / 2 : 1; }
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
+  s = t;
+}
+
+// Reduce the partials of one step into its history row (one workgroup).
+template <int NT>
+__device__ void finalize_block(const FinalArgs& f, double* scratch) {
+  double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+  double mn = INFINITY, mx = -INFINITY, bad = 0.0;
+  for (int i = threadIdx.x; i < f.n1; i += NT) {
+    const double* r = f.p1 + 5 * i;
+    neumaier(s[0], c[0], r[0]);
+    neumaier(s[1], c[1], r[1]);
+    mn = red_combine<RED_MIN>(mn, r[2]);
+    mx = red_combine<RED_MAX>(mx, r[3]);
+    bad += r[4];
   }
-  double r = red[0];
-  __syncthreads();
-  return r;
+  for (int i = threadIdx.x; i < f.n2; i += NT) neumaier(s[2], c[2], f.p2[i]);
+  for (int i = threadIdx.x; i < f.n3; i += NT) neumaier(s[3], c[3], f.p3[i]);
+  double v[7] = {s[0] + c[0], s[1] + c[1], s[2] + c[2], s[3] + c[3], mn, mx, bad};
+  const int ops[7] = {RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MIN, RED_MAX, RED_SUM};
+  block_reduce<NT, 7>(v, ops, scratch);
+  if (threadIdx.x == 0) {
+    const long long row = f.ctr[0];
+    double* h = f.hist + 6 * row;
+    const bool nonfinite = v[6] > 0 || v[4] != v[4] || v[5] != v[5];
+    h[0] = static_cast<double>(row) * f.dt;
+    h[1] = v[1] * f.inv_n2;
+    h[2] = (v[0] + f.kappa * v[2] * f.inv_n2) * f.spacing2;   // core_shell.jl:80-81
+    h[3] = nonfinite ? NAN : v[4];
+    h[4] = nonfinite ? NAN : v[5];
+    h[5] = v[3] * f.spacing2;                                  // core_shell.jl:68
+    if (nonfinite && f.ctr[1] < 0) f.ctr[1] = row;
+    f.ctr[0] = row + 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Row pass.  CT: compute/LDS type; TIN/TOUT: element types of the packed
-// input/output arrays; the real rows (FWD input, INV output) are double.
+// Row pass: B rows of L = N/2 packed complex per workgroup, TPL threads per
+// row.  CT: compute type; TIN/TOUT: packed input/output element types; the
+// real rows (FWD input, INV output) are double.
+// Rows per workgroup: as many as fit an LDS budget (more rows make the
+// transposed accesses to the line layout wider), up to 8 and 1024 threads.
+#ifndef CSG_ROW_LDS
+#define CSG_ROW_LDS (40 * 1024)
+#endif
+template <int N, class CT, int NARR>
+constexpr int row_rows() {
+  constexpr int L = N / 2, TPL = L / fft_E(L);
+  const int lp = NARR * padded_len<CT>(L) * 2 * (int)sizeof(CT);
+  int b = 1;
+  while (b < 8 && b * 2 <= N && (b * 2) * lp <= CSG_ROW_LDS && (b * 2) * TPL <= 1024) b *= 2;
+  while (b * TPL < 64 && b * 2 <= N) b *= 2;
+  return b;
+}
+
+// P3 gives each gradient component its own workgroups (CSG_P3_SPLIT=1,
+// more parallelism) or transforms both in one workgroup (phi read once).
+#ifndef CSG_P3_SPLIT
+#define CSG_P3_SPLIT 1
+#endif
+
 template <class CT, class TIN, class TOUT, int N, int MODE>
 struct RowCfg {
   static constexpr int L = N / 2;
-  static constexpr int NARR = (MODE == ROW_P3) ? 2 : 1;
-  static constexpr int B = row_B<N, sizeof(CT), NARR>();
-  static constexpr int PITCH = L + 1;
-  static constexpr int NLINES = NARR * B;
-  static constexpr int NT_FFT = NLINES * L / 8;
-  static constexpr int NT = NT_FFT < 64 ? 64 : NT_FFT;
-  static constexpr int GRID = N / B;
+  static constexpr int E = fft_E(L);
+  static constexpr int TPL = L / E;
+  static constexpr bool SPLIT = (MODE == ROW_P3) && CSG_P3_SPLIT;
+  static constexpr int NARR = (MODE == ROW_P3 && !SPLIT) ? 2 : 1;   // arrays per workgroup
+  static constexpr int B = row_rows<N, CT, NARR>();
+  static constexpr int LP = padded_len<CT>(L);
+  static constexpr int NT = B * TPL;
+  static constexpr bool WAVE = TPL <= 64;
+  static constexpr int TILES = N / B;
+  static constexpr int GRID = SPLIT ? 2 * TILES : TILES;
+  static_assert(NT % 64 == 0 || NT < 64, "workgroup of whole wavefronts");
 };
 
 template <class CT, class TIN, class TOUT, int N, int MODE>
 __global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE>::NT))
-row_pass(PassArgs a, const cplx<CT>* __restrict__ tw) {
+row_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
-  constexpr int L = Cfg::L, B = Cfg::B, PITCH = Cfg::PITCH, NARR = Cfg::NARR;
-  constexpr int NLINES = Cfg::NLINES, NT = Cfg::NT;
-  __shared__ cplx<CT> lds[NLINES * PITCH];
-  __shared__ double red[NT];
-  const int j0 = blockIdx.x * B;
+  constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
+  constexpr int NARR = Cfg::NARR;
+  constexpr bool WAVE = Cfg::WAVE;
+  __shared__ cplx<CT> lds[NARR * B * LP];
+  __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
+  __shared__ double scratch[8 * ((NT + 63) / 64)];
+  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;
+  // split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2 TILES) take Gy/Jy
+  const int tile = xcd_tile(blockIdx.x, Cfg::GRID);
+  const int arr0 = (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0;
+  const int j0 = (tile - arr0 * Cfg::TILES) * B;
+  const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  cplx<CT>* row = lds + b * LP;                 // FFT exchange buffer of this row
+  const size_t rbase = (size_t)(j0 + b) * N;   // real-row offset
+  cplx<CT> v[NARR][E];
+  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // partials, reduced at the end
 
+  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
   if constexpr (MODE == ROW_FWD) {
-    // real rows -> LDS (row b, as L complex = interleaved reals)
     const double* x = static_cast<const double*>(a.in0);
-    for (int f = threadIdx.x; f < B * L; f += NT) {
-      const int b = f / L, n = f - b * L;
-      const double* r = x + (size_t)(j0 + b) * N + 2 * n;
-      lds[b * PITCH + n] = mk(static_cast<CT>(r[0]), static_cast<CT>(r[1]));
-    }
-  } else {
-    // transposed tile load from line layout: lane-consecutive b (coalesced)
 #pragma unroll
-    for (int arr = 0; arr < NARR; ++arr) {
-      const cplx<TIN>* X = static_cast<const cplx<TIN>*>(arr == 0 ? a.in0 : a.in1);
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
+    }
+    __syncthreads();
+  } else {
+    // transposed tile load from line layout: lane-consecutive rows (coalesced)
+#pragma unroll
+    for (int q = 0; q < NARR; ++q) {
+      const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
       for (int f = threadIdx.x; f < B * L; f += NT) {
-        const int b = f % B, kx = f / B;
-        lds[(arr * B + b) * PITCH + kx] = cvt<CT>(X[(size_t)kx * N + j0 + b]);
+        const int bb = f % B, kx = f / B;
+        lds[(q * B + bb) * LP + pad<CT>(kx)] = cvt<CT>(X[(size_t)kx * N + j0 + bb]);
       }
     }
-  }
-  __syncthreads();
-
-  if constexpr (MODE != ROW_FWD) {
-    // C2R along i: packed -> real row (interleaved in place), includes 1/N
-    c2r_pre<CT, N, NLINES, PITCH, NT>(lds, tw, static_cast<CT>(1.0 / N));
-    fft_lines<CT, +1, L, NLINES, PITCH, NT, 2>(lds, tw);
+    __syncthreads();
+    // C2R pre-processing fused with the read: Z_k = (X_k + conj X_{L-k}) +
+    // i (X_k - conj X_{L-k}) W_N^{-k}, times 1/N; slot 0 holds (X_0, X_L).
+    const CT scale = static_cast<CT>(1.0 / N);
+#pragma unroll
+    for (int q = 0; q < NARR; ++q) {
+      const cplx<CT>* tr = lds + (q * B + b) * LP;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int k = t + TPL * j;
+        const cplx<CT> xk = tr[pad<CT>(k)];
+        if (k == 0) {
+          v[q][j] = mk(scale * (xk.x + xk.y), scale * (xk.x - xk.y));
+        } else {
+          const cplx<CT> xm = tr[pad<CT>(L - k)];
+          v[q][j] = scale * ((xk + conj(xm)) + mul_i(cmul(xk - conj(xm), conj(tw[k]))));
+        }
+      }
+    }
+    lds_sync<WAVE>();
+    fft_regs<CT, +1, L, NARR, N, WAVE>(v, t, row, tw);
   }
 
   if constexpr (MODE == ROW_INV) {
     double* x = static_cast<double*>(a.out0);
-    for (int f = threadIdx.x; f < B * L; f += NT) {
-      const int b = f / L, n = f - b * L;
-      const cplx<CT> v = lds[b * PITCH + n];
-      double* r = x + (size_t)(j0 + b) * N + 2 * n;
-      r[0] = static_cast<double>(v.x);
-      r[1] = static_cast<double>(v.y);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      x[rbase + 2 * k] = static_cast<double>(v[0][j].x);
+      x[rbase + 2 * k + 1] = static_cast<double>(v[0][j].y);
     }
     return;
   }
 
   if constexpr (MODE == ROW_P1) {
-    // pointwise: f'(c) in place; bulk energy, min, max, non-finite partials
+    // pointwise: f'(c); bulk energy, min, max, non-finite partials
     const CT* phi = static_cast<const CT*>(a.phi);
     const DevParams& p = a.p;
     const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
-    double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY;
-    int bad = 0;
-    for (int f = threadIdx.x; f < B * L; f += NT) {
-      const int b = f / L, n = f - b * L;
-      cplx<CT> v = lds[b * PITCH + n];
-      const CT* ph = phi + (size_t)(j0 + b) * N + 2 * n;
+    double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      const CT cin[2] = {v[0][j].x, v[0][j].y};
+      const CT qq[2] = {phi[rbase + 2 * k], phi[rbase + 2 * k + 1]};
       CT out[2];
-      CT cin[2] = {v.x, v.y};
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        const CT c = cin[e], q = ph[e];
+        const CT c = cin[e], q = qq[e];
         const CT h = q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q);   // core_shell.jl:40
         const CT g = q * q * (CT(1) - q) * (CT(1) - q);                     // :41
         // bulk_derivative, core_shell.jl:53-54
@@ -204,195 +273,242 @@ row_pass(PassArgs a, const cplx<CT>* __restrict__ tw) {
         const CT fb = A * (CT(1) - h) * (c - ca) * (c - ca) +
                       Bc * h * (c - cb) * (c - cb) * (c - cc) * (c - cc) + P * (CT(1) - chi * c) * g;
         fsum += static_cast<double>(fb);
+        // branchless: non-finite values are counted and kept out of min/max
+        const bool ok = finite(c);
         const double cd = static_cast<double>(c);
-        cmin = fmin(cmin, cd);
-        cmax = fmax(cmax, cd);
-        bad |= !finite(c);
+        cmin = fmin(cmin, ok ? cd : INFINITY);
+        cmax = fmax(cmax, ok ? cd : -INFINITY);
+        nbad += ok ? 0.0 : 1.0;
       }
-      lds[b * PITCH + n] = mk(out[0], out[1]);
+      v[0][j] = mk(out[0], out[1]);
     }
-    // mean composition from the FP64 k_x = 0 row sums of the state
-    double msum = 0.0;
-    if (threadIdx.x < B) {
-      const cplx<TIN>* S = static_cast<const cplx<TIN>*>(a.in0);
-      msum = static_cast<double>(S[j0 + threadIdx.x].x);
-    }
-    __syncthreads();
-    fsum = block_sum<NT>(fsum, red);
-    msum = block_sum<NT>(msum, red);
-    cmin = block_minmax<NT, false>(bad ? NAN : cmin, red);
-    cmax = block_minmax<NT, true>(bad ? NAN : cmax, red);
-    const double nbad = block_sum<NT>(bad ? 1.0 : 0.0, red);
-    if (threadIdx.x == 0) {
-      double* rec = a.part + 5 * blockIdx.x;
-      rec[0] = fsum;
-      rec[1] = msum;
-      rec[2] = cmin;
-      rec[3] = cmax;
-      rec[4] = nbad;
-    }
+    red[0] = fsum;
+    red[2] = cmin;
+    red[3] = cmax;
+    red[4] = nbad;
   }
 
   if constexpr (MODE == ROW_P3) {
-    // J = M0 h(phi) g for both gradient components; dissipation partial
+    // J = M0 h(phi) g for the gradient component(s); dissipation partial
     const CT* phi = static_cast<const CT*>(a.phi);
     const CT mob = a.p.mobility;
     double dsum = 0.0;
-    for (int f = threadIdx.x; f < B * L; f += NT) {
-      const int b = f / L, n = f - b * L;
-      cplx<CT> gx = lds[b * PITCH + n];
-      cplx<CT> gy = lds[(B + b) * PITCH + n];
-      const CT* ph = phi + (size_t)(j0 + b) * N + 2 * n;
-      CT M[2];
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const CT q = ph[e];
-        M[e] = mob * (q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q));   // :42
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      const CT q0 = phi[rbase + 2 * k], q1 = phi[rbase + 2 * k + 1];
+      const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
+      const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
+#pragma unroll
+      for (int q = 0; q < NARR; ++q) {
+        const cplx<CT> g = v[q][j];
+        // core_shell.jl:68 sum(M .* (gx.^2 .+ gy.^2))
+        dsum += static_cast<double>(M0 * (g.x * g.x)) + static_cast<double>(M1 * (g.y * g.y));
+        v[q][j] = mk(M0 * g.x, M1 * g.y);
       }
-      // core_shell.jl:68 sum(M .* (gx.^2 .+ gy.^2))
-      dsum += static_cast<double>(M[0] * (gx.x * gx.x + gy.x * gy.x)) +
-              static_cast<double>(M[1] * (gx.y * gx.y + gy.y * gy.y));
-      lds[b * PITCH + n] = mk(M[0] * gx.x, M[1] * gx.y);
-      lds[(B + b) * PITCH + n] = mk(M[0] * gy.x, M[1] * gy.y);
     }
-    __syncthreads();
-    dsum = block_sum<NT>(dsum, red);
-    if (threadIdx.x == 0) a.part[blockIdx.x] = dsum;
+    red[0] = dsum;
   }
 
-  if constexpr (MODE == ROW_P1 || MODE == ROW_P3) {
-    // (block_sum ended with a barrier)
-  } else {
-    __syncthreads();
-  }
-
-  // R2C along i and transposed store
-  fft_lines<CT, -1, L, NLINES, PITCH, NT, 2>(lds, tw);
-  r2c_post<CT, N, NLINES, PITCH, NT>(lds, tw);
+  // R2C along i: forward FFT, then X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k})
+  fft_regs<CT, -1, L, NARR, N, WAVE>(v, t, row, tw);
 #pragma unroll
-  for (int arr = 0; arr < NARR; ++arr) {
-    cplx<TOUT>* X = static_cast<cplx<TOUT>*>(arr == 0 ? a.out0 : a.out1);
-    for (int f = threadIdx.x; f < B * L; f += NT) {
-      const int b = f % B, kx = f / B;
-      X[(size_t)kx * N + j0 + b] = cvt<TOUT>(lds[(arr * B + b) * PITCH + kx]);
+  for (int q = 0; q < NARR; ++q) {
+    cplx<CT>* tr = lds + (q * B + b) * LP;
+#pragma unroll
+    for (int j = 0; j < E; ++j) tr[pad<CT>(t + TPL * j)] = v[q][j];
+  }
+  lds_sync<WAVE>();
+  const CT half = static_cast<CT>(0.5);
+#pragma unroll
+  for (int q = 0; q < NARR; ++q) {
+    const cplx<CT>* tr = lds + (q * B + b) * LP;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      const cplx<CT> zk = v[q][j];
+      if (k == 0) {
+        v[q][j] = mk(zk.x + zk.y, zk.x - zk.y);
+      } else {
+        const cplx<CT> zm = tr[pad<CT>(L - k)];
+        const cplx<CT> ek = half * (zk + conj(zm));
+        const cplx<CT> ok = half * mul_mi(zk - conj(zm));
+        v[q][j] = ek + cmul(tw[k], ok);
+      }
     }
+  }
+  lds_sync<WAVE>();
+#pragma unroll
+  for (int q = 0; q < NARR; ++q) {
+    cplx<CT>* tr = lds + (q * B + b) * LP;
+#pragma unroll
+    for (int j = 0; j < E; ++j) tr[pad<CT>(t + TPL * j)] = v[q][j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NARR; ++q) {
+    cplx<TOUT>* X = static_cast<cplx<TOUT>*>((arr0 + q) == 0 ? a.out0 : a.out1);
+    for (int f = threadIdx.x; f < B * L; f += NT) {
+      const int bb = f % B, kx = f / B;
+      X[(size_t)kx * N + j0 + bb] = cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT>(kx)]);
+    }
+  }
+
+  if constexpr (MODE == ROW_P1) {
+    // mean composition from the FP64 k_x = 0 row sums of the state
+    if (threadIdx.x < B) red[1] = static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x);
+    const int ops[5] = {RED_SUM, RED_SUM, RED_MIN, RED_MAX, RED_SUM};
+    block_reduce<NT, 5>(red, ops, scratch);
+    if (threadIdx.x == 0) {
+      double* rec = a.part + 5 * blockIdx.x;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) rec[q] = red[q];
+    }
+  }
+  if constexpr (MODE == ROW_P3) {
+    double r[1] = {red[0]};
+    const int ops[1] = {RED_SUM};
+    block_reduce<NT, 1>(r, ops, scratch);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = r[0];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Column pass.  Lines kx0 .. kx0+C-1 of length N.
+// Column pass: C lines kx0 .. kx0+C-1 of length N per workgroup, TPL threads
+// per line; each thread group transforms both input arrays of its line.
+template <int N, class CT>
+constexpr int col_lines() {
+  constexpr int TPL = N / fft_E(N);
+  if (TPL > 64) return 1;   // lines spanning wavefronts: one per workgroup
+  const int lp = padded_len<CT>(N) * 2 * (int)sizeof(CT);
+  int c = 1;
+  while (c * 2 <= N / 2 && (c * 2) * lp <= 40 * 1024 && (c * 2) * TPL <= 256) c *= 2;
+  while (c * TPL < 64 && c * 2 <= N / 2) c *= 2;
+  return c;
+}
+
 template <class CT, class TS, int N, int MODE>
 struct ColCfg {
-  static constexpr int C = col_C<N, sizeof(CT)>();
-  static constexpr int PITCH = N;
-  static constexpr int NLINES = 2 * C;
-  static constexpr int NT_FFT = NLINES * N / 8;
-  static constexpr int NT = NT_FFT < 64 ? 64 : NT_FFT;
+  static constexpr int E = fft_E(N);
+  static constexpr int TPL = N / E;
+  static constexpr int C = col_lines<N, CT>();
+  static constexpr int LP = padded_len<CT>(N);
+  static constexpr int NT = C * TPL;
+  static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = (N / 2) / C;
+  static_assert(WAVE || C == 1, "block-synchronised lines need one line per workgroup");
 };
 
 template <class CT, class TS, int N, int MODE>
 __global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE>::NT))
-col_pass(PassArgs a, const cplx<CT>* __restrict__ tw) {
+col_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
   using Cfg = ColCfg<CT, TS, N, MODE>;
-  constexpr int C = Cfg::C, PITCH = Cfg::PITCH, NLINES = Cfg::NLINES, NT = Cfg::NT;
-  constexpr int PER = (C * N + NT - 1) / NT;  // k-space elements per thread
-  __shared__ cplx<CT> lds[NLINES * PITCH];
-  __shared__ double red[NT];
-  const int kx0 = blockIdx.x * C;
+  constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
+  constexpr bool WAVE = Cfg::WAVE;
+  __shared__ cplx<CT> lds[C * LP];
+  __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
+  __shared__ double scratch[8 * ((NT + 63) / 64)];
+  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;
+  const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const int kx = blockIdx.x * C + l;
+  cplx<CT>* line = lds + l * LP;
+  const size_t base = (size_t)kx * N;
   const DevParams& p = a.p;
   const double inv_nsp = p.inv_nsp;
+  const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);  // (pi/spacing)^2
 
-  // load: lines [0,C) <- in0, [C,2C) <- in1 (contiguous)
-  {
-    const size_t base = (size_t)kx0 * N;
-    if constexpr (MODE == COL_P2 || MODE == COL_MU) {
-      const cplx<TS>* S = static_cast<const cplx<TS>*>(a.in0);
-      const cplx<CT>* F = static_cast<const cplx<CT>*>(a.in1);
-      for (int f = threadIdx.x; f < C * N; f += NT) {
-        lds[f] = cvt<CT>(S[base + f]);
-        lds[C * PITCH + f] = F[base + f];
-      }
-    } else {
-      const cplx<CT>* X0 = static_cast<const cplx<CT>*>(a.in0);
-      const cplx<CT>* X1 = static_cast<const cplx<CT>*>(a.in1);
-      for (int f = threadIdx.x; f < C * N; f += NT) {
-        lds[f] = X0[base + f];
-        lds[C * PITCH + f] = X1[base + f];
-      }
+  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+  cplx<CT> v[2][E];
+  if constexpr (MODE == COL_P2 || MODE == COL_MU) {
+    const cplx<TS>* S = static_cast<const cplx<TS>*>(a.in0);
+    const cplx<CT>* F = static_cast<const cplx<CT>*>(a.in1);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      v[0][j] = cvt<CT>(S[base + t + TPL * j]);
+      v[1][j] = F[base + t + TPL * j];
+    }
+  } else {
+    const cplx<CT>* X0 = static_cast<const cplx<CT>*>(a.in0);
+    const cplx<CT>* X1 = static_cast<const cplx<CT>*>(a.in1);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      v[0][j] = X0[base + t + TPL * j];
+      v[1][j] = X1[base + t + TPL * j];
     }
   }
   __syncthreads();
-  fft_lines<CT, -1, N, NLINES, PITCH, NT, 1>(lds, tw);
+  fft_regs<CT, -1, N, 2, N, WAVE>(v, t, line, tw);
 
-  const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);  // (pi/spacing)^2
+  // packed line 0 (k_x = 0 in re, k_x = N/2 in im) needs the value at -ky of
+  // one array: stage it through the line buffer
+  cplx<CT> partner[E];
+  if (kx == 0) {
+    constexpr int PA = (MODE == COL_P2 || MODE == COL_MU) ? 0 : 1;
+    lds_sync<WAVE>();
+#pragma unroll
+    for (int j = 0; j < E; ++j) line[pad<CT>(t + TPL * j)] = v[PA][j];
+    lds_sync<WAVE>();
+#pragma unroll
+    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT>((N - (t + TPL * j)) & (N - 1))];
+    lds_sync<WAVE>();
+  }
 
   if constexpr (MODE == COL_P2 || MODE == COL_MU) {
-    cplx<CT> o0[PER], o1[PER];
     double esum = 0.0;
     const double tk = 2.0 * p.kappa;
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int f = threadIdx.x + t * NT;
-      if (C * N % NT == 0 || f < C * N) {
-        const int l = f / N, ky = f - l * N;
-        const int kx = kx0 + l;
-        const cplx<CT> ch = lds[l * PITCH + ky];
-        const cplx<CT> fh = lds[(C + l) * PITCH + ky];
-        const double kyv = kval<N>(ky, inv_nsp);
-        const double ky2 = kyv * kyv;
-        cplx<CT> mu;
-        if (kx == 0) {
-          // packed line: A part is k_x = 0, B part is k_x = N/2 (Nyquist, k^2 kept)
-          const cplx<CT> chn = lds[l * PITCH + ((N - ky) & (N - 1))];
-          const CT sa = static_cast<CT>(tk * (ky2 + 0.5 * kN2));   // (a+b)/2
-          const CT sd = static_cast<CT>(-0.5 * tk * kN2);           // (a-b)/2
-          mu = fh + sa * ch + sd * conj(chn);
-          // |A|^2 K_A^2 + |B|^2 K_B^2 summed over ky, written per element
-          const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
-          const double re = (double)ch.x * chn.x - (double)ch.y * chn.y;
-          esum += (ky2 + 0.5 * kN2) * m2 - 0.5 * kN2 * re;
-        } else {
-          const double kxv = kval<N>(kx, inv_nsp);
-          const double k2 = kxv * kxv + ky2;   // core_shell.jl:23
-          mu = fh + static_cast<CT>(tk * k2) * ch;
-          const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
-          esum += 2.0 * k2 * m2;   // Hermitian weight of interior k_x
-        }
-        if constexpr (MODE == COL_MU) {
-          o0[t] = static_cast<CT>(1.0 / N) * mu;
-        } else {
-          const CT sx = static_cast<CT>(dkval<N>(kx, inv_nsp) / N);   // 0 on line 0
-          const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp) / N);
-          o0[t] = mul_i(sx * mu);
-          o1[t] = mul_i(sy * mu);
-        }
+    for (int j = 0; j < E; ++j) {
+      const int ky = t + TPL * j;
+      const cplx<CT> ch = v[0][j], fh = v[1][j];
+      const double kyv = kval<N>(ky, inv_nsp);
+      const double ky2 = kyv * kyv;
+      cplx<CT> mu;
+      if (kx == 0) {
+        // A part: k_x = 0; B part: k_x = N/2 (Nyquist, kept in k^2)
+        const cplx<CT> chn = partner[j];
+        const CT sa = static_cast<CT>(tk * (ky2 + 0.5 * kN2));   // (a+b)/2
+        const CT sd = static_cast<CT>(-0.5 * tk * kN2);           // (a-b)/2
+        mu = fh + sa * ch + sd * conj(chn);
+        const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
+        const double re = (double)ch.x * chn.x - (double)ch.y * chn.y;
+        esum += (ky2 + 0.5 * kN2) * m2 - 0.5 * kN2 * re;   // K_A^2|A|^2 + K_B^2|B|^2
+      } else {
+        const double kxv = kval<N>(kx, inv_nsp);
+        const double k2 = kxv * kxv + ky2;   // core_shell.jl:23
+        mu = fh + static_cast<CT>(tk * k2) * ch;
+        const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
+        esum += 2.0 * k2 * m2;   // Hermitian weight of interior k_x
+      }
+      if constexpr (MODE == COL_MU) {
+        v[0][j] = static_cast<CT>(1.0 / N) * mu;
+      } else {
+        const CT sx = static_cast<CT>(dkval<N>(kx, inv_nsp) / N);   // 0 on line 0
+        const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp) / N);
+        v[0][j] = mul_i(sx * mu);
+        v[1][j] = mul_i(sy * mu);
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int f = threadIdx.x + t * NT;
-      if (C * N % NT == 0 || f < C * N) {
-        const int l = f / N, ky = f - l * N;
-        lds[l * PITCH + ky] = o0[t];
-        if constexpr (MODE == COL_P2) lds[(C + l) * PITCH + ky] = o1[t];
-      }
-    }
-    __syncthreads();
     if constexpr (MODE == COL_P2) {
-      fft_lines<CT, +1, N, NLINES, PITCH, NT, 1>(lds, tw);
-    } else {
-      fft_lines<CT, +1, N, C, PITCH, NT, 1>(lds, tw);
-    }
-    const size_t base = (size_t)kx0 * N;
-    cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
-    for (int f = threadIdx.x; f < C * N; f += NT) O0[base + f] = lds[f];
-    if constexpr (MODE == COL_P2) {
+      fft_regs<CT, +1, N, 2, N, WAVE>(v, t, line, tw);
+      cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
       cplx<CT>* O1 = static_cast<cplx<CT>*>(a.out1);
-      for (int f = threadIdx.x; f < C * N; f += NT) O1[base + f] = lds[C * PITCH + f];
-      esum = block_sum<NT>(esum, red);
-      if (threadIdx.x == 0) a.part[blockIdx.x] = esum;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        O0[base + t + TPL * j] = v[0][j];
+        O1[base + t + TPL * j] = v[1][j];
+      }
+      double r[1] = {esum};
+      const int ops[1] = {RED_SUM};
+      block_reduce<NT, 1>(r, ops, scratch);
+      if (threadIdx.x == 0) a.part[blockIdx.x] = r[0];
+    } else {
+      cplx<CT> w[1][E];
+#pragma unroll
+      for (int j = 0; j < E; ++j) w[0][j] = v[0][j];
+      fft_regs<CT, +1, N, 1, N, WAVE>(w, t, line, tw);
+      cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
+#pragma unroll
+      for (int j = 0; j < E; ++j) O0[base + t + TPL * j] = w[0][j];
     }
     return;
   }
@@ -400,83 +516,59 @@ col_pass(PassArgs a, const cplx<CT>* __restrict__ tw) {
   if constexpr (MODE == COL_P4 || MODE == COL_RATE) {
     const double dt = a.dt;
     const double qk2dt = 2.0 * p.Q * p.kappa;   // core_shell.jl:74  2p.Q*p.kappa*k2^2*dt
-    cplx<CT> o0[PER], o1[PER];
+    cplx<CT> w[1][E];
+    cplx<double>* R = static_cast<cplx<double>*>(a.out0);
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int f = threadIdx.x + t * NT;
-      if (C * N % NT == 0 || f < C * N) {
-        const int l = f / N, ky = f - l * N;
-        const int kx = kx0 + l;
-        const cplx<CT> jx = lds[l * PITCH + ky];
-        const cplx<CT> jy = lds[(C + l) * PITCH + ky];
-        const double kyv = kval<N>(ky, inv_nsp);
-        const double ky2 = kyv * kyv;
-        const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp));
-        if (kx == 0) {
-          const int kn = (N - ky) & (N - 1);
-          const cplx<CT> jyn = lds[(C + l) * PITCH + kn];
-          const CT syn = static_cast<CT>(dkval<N>(kn, inv_nsp));
-          // rate on the packed line: i*kdy*Jy^ (k_dx = 0 for k_x = 0 and N/2)
-          const cplx<CT> r = mul_i(sy * jy);
-          const cplx<CT> rn = mul_i(syn * jyn);
-          if constexpr (MODE == COL_RATE) {
-            // unpack: A = (r + conj rn)/2 (k_x = 0), B = (r - conj rn)/(2i) (k_x = N/2)
-            o0[t] = static_cast<CT>(0.5) * (r + conj(rn));
-            o1[t] = static_cast<CT>(0.5) * mul_mi(r - conj(rn));
-          } else {
-            const double k2a = ky2, k2b = kN2 + ky2;
-            const double ia = 1.0 / (1.0 + qk2dt * (k2a * k2a) * dt);
-            const double ib = 1.0 / (1.0 + qk2dt * (k2b * k2b) * dt);
-            const CT s = static_cast<CT>(dt / N);
-            const CT ha = static_cast<CT>(0.5 * (ia + ib)), hd = static_cast<CT>(0.5 * (ia - ib));
-            o0[t] = s * (ha * r + hd * conj(rn));
-          }
+    for (int j = 0; j < E; ++j) {
+      const int ky = t + TPL * j;
+      const cplx<CT> jx = v[0][j], jy = v[1][j];
+      const double kyv = kval<N>(ky, inv_nsp);
+      const double ky2 = kyv * kyv;
+      const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp));
+      if (kx == 0) {
+        const int kn = (N - ky) & (N - 1);
+        const CT syn = static_cast<CT>(dkval<N>(kn, inv_nsp));
+        // rate on the packed line: i*kdy*Jy^ (k_dx = 0 for k_x = 0 and N/2)
+        const cplx<CT> r = mul_i(sy * jy);
+        const cplx<CT> rn = mul_i(syn * partner[j]);
+        if constexpr (MODE == COL_RATE) {
+          // unpack: A = (r + conj rn)/2 (k_x = 0), B = (r - conj rn)/(2i) (k_x = N/2)
+          R[ky] = cvt<double>(static_cast<CT>(0.5) * (r + conj(rn)));
+          R[(size_t)(N / 2) * N + ky] = cvt<double>(static_cast<CT>(0.5) * mul_mi(r - conj(rn)));
         } else {
-          const double kxv = kval<N>(kx, inv_nsp);
-          const CT sx = static_cast<CT>(kxv);   // interior k_x: no Nyquist
-          const cplx<CT> r = mul_i(sx * jx) + mul_i(sy * jy);
-          if constexpr (MODE == COL_RATE) {
-            o0[t] = r;
-          } else {
-            const double k2 = kxv * kxv + ky2;
-            const double den = 1.0 + qk2dt * (k2 * k2) * dt;
-            o0[t] = static_cast<CT>(dt / (den * N)) * r;
-          }
+          const double k2a = ky2, k2b = kN2 + ky2;
+          const double ia = 1.0 / (1.0 + qk2dt * (k2a * k2a) * dt);
+          const double ib = 1.0 / (1.0 + qk2dt * (k2b * k2b) * dt);
+          const CT s = static_cast<CT>(dt / N);
+          const CT ha = static_cast<CT>(0.5 * (ia + ib)), hd = static_cast<CT>(0.5 * (ia - ib));
+          w[0][j] = s * (ha * r + hd * conj(rn));
+        }
+      } else {
+        const double kxv = kval<N>(kx, inv_nsp);
+        const CT sx = static_cast<CT>(kxv);   // interior k_x: no Nyquist
+        const cplx<CT> r = mul_i(sx * jx) + mul_i(sy * jy);
+        if constexpr (MODE == COL_RATE) {
+          R[base + ky] = cvt<double>(r);
+        } else {
+          const double k2 = kxv * kxv + ky2;
+          const double den = 1.0 + qk2dt * (k2 * k2) * dt;
+          w[0][j] = static_cast<CT>(dt / (den * N)) * r;
         }
       }
     }
-    if constexpr (MODE == COL_RATE) {
-      // k-space output, k_x = 0 .. N/2 lines of N (double): line kx at kx*N
-      cplx<double>* R = static_cast<cplx<double>*>(a.out0);
+    if constexpr (MODE == COL_P4) {
+      fft_regs<CT, +1, N, 1, N, WAVE>(w, t, line, tw);
+      // S += IFFT_j(inc)
+      cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
 #pragma unroll
-      for (int t = 0; t < PER; ++t) {
-        const int f = threadIdx.x + t * NT;
-        if (C * N % NT == 0 || f < C * N) {
-          const int l = f / N, ky = f - l * N;
-          const int kx = kx0 + l;
-          R[(size_t)kx * N + ky] = cvt<double>(o0[t]);
-          if (kx == 0) R[(size_t)(N / 2) * N + ky] = cvt<double>(o1[t]);
-        }
+      for (int j = 0; j < E; ++j) {
+        cplx<TS> s = S[base + t + TPL * j];
+        s.x += static_cast<TS>(w[0][j].x);
+        s.y += static_cast<TS>(w[0][j].y);
+        S[base + t + TPL * j] = s;
       }
-      return;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int f = threadIdx.x + t * NT;
-      if (C * N % NT == 0 || f < C * N) lds[f] = o0[t];
-    }
-    __syncthreads();
-    fft_lines<CT, +1, N, C, PITCH, NT, 1>(lds, tw);
-    // S += IFFT_j(inc)
-    cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
-    const size_t base = (size_t)kx0 * N;
-    for (int f = threadIdx.x; f < C * N; f += NT) {
-      cplx<TS> s = S[base + f];
-      const cplx<CT> d = lds[f];
-      s.x += static_cast<TS>(d.x);
-      s.y += static_cast<TS>(d.y);
-      S[base + f] = s;
+      // the partials of P1..P3 of this step are complete: history row
+      if (blockIdx.x == 0 && a.fin.hist != nullptr) finalize_block<NT>(a.fin, scratch);
     }
   }
 }
@@ -510,71 +602,11 @@ pointwise_kernel(const double* __restrict__ c, const TP* __restrict__ phi, doubl
   }
 }
 
-// ---------------------------------------------------------------------------
-// Reduction of the per-workgroup partials into one history row
-// [t, mean c, F, min c, max c, dissipation] (core_shell.jl:92), FP64 with
-// Neumaier-compensated sums in a fixed order.
-struct FinalArgs {
-  const double* p1;   // 5 per row-pass workgroup
-  const double* p2;   // 1 per column-pass workgroup
-  const double* p3;   // 1 per row-pass workgroup
-  int n1, n2, n3;
-  double* hist;       // base of the history buffer (6 per row)
-  long long* ctr;     // [0] = row index of this step, [1] = first non-finite step (-1)
-  double dt;
-  double spacing2;
-  double inv_n2;      // 1 / N^2
-  double kappa;
-  long long step0;    // step number of row 0
-};
-
-__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
-  double t = s + v;
-  if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
-  s = t;
-}
-
-template <int NT>
-__device__ void finalize_block(const FinalArgs& f, double* red) {
-  // each thread sums a strided slice (fixed order), then a fixed tree
-  double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
-  double mn = INFINITY, mx = -INFINITY, bad = 0.0;
-  for (int i = threadIdx.x; i < f.n1; i += NT) {
-    const double* r = f.p1 + 5 * i;
-    neumaier(s[0], c[0], r[0]);
-    neumaier(s[1], c[1], r[1]);
-    const double a = r[2], b = r[3];
-    if (a != a || b != b) bad += 1.0; else { mn = fmin(mn, a); mx = fmax(mx, b); }
-    bad += r[4];
-  }
-  for (int i = threadIdx.x; i < f.n2; i += NT) neumaier(s[2], c[2], f.p2[i]);
-  for (int i = threadIdx.x; i < f.n3; i += NT) neumaier(s[3], c[3], f.p3[i]);
-  double tot[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) tot[q] = block_sum<NT>(s[q], red) + block_sum<NT>(c[q], red);
-  mn = block_minmax<NT, false>(mn, red);
-  mx = block_minmax<NT, true>(mx, red);
-  bad = block_sum<NT>(bad, red);
-  if (threadIdx.x == 0) {
-    const long long row = f.ctr[0];
-    const long long step = f.step0 + row;
-    double* h = f.hist + 6 * row;
-    h[0] = static_cast<double>(step) * f.dt;
-    h[1] = tot[1] * f.inv_n2;
-    // energy, core_shell.jl:80-81
-    h[2] = (tot[0] + f.kappa * tot[2] * f.inv_n2) * f.spacing2;
-    h[3] = bad > 0 ? NAN : mn;
-    h[4] = bad > 0 ? NAN : mx;
-    h[5] = tot[3] * f.spacing2;
-    if (bad > 0 && f.ctr[1] < 0) f.ctr[1] = step;
-    f.ctr[0] = row + 1;
-  }
-}
-
+// Standalone history reduction (the last row of a run, single evaluations).
 template <int NT>
 __global__ void __launch_bounds__(NT) finalize_kernel(FinalArgs f) {
-  __shared__ double red[NT];
-  finalize_block<NT>(f, red);
+  __shared__ double scratch[8 * ((NT + 63) / 64)];
+  finalize_block<NT>(f, scratch);
 }
 
 }  // namespace csg

@@ -1,0 +1,103 @@
+// Host-only check of the register Stockham engine (csg_fft.hpp): drives the
+// engine's stage_compute / plan / padding / Stockham output indexing over
+// emulated threads of one line and compares forward and inverse transforms
+// with a naive DFT for every line length the kernels use.  No GPU needed.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "../../embedded-spinodal-core-shell-julia_amd/csrc/csg_fft.hpp"
+
+using namespace csg;
+
+template <int DIR, int LEN, int S, class T>
+void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::vector<cplx<T>>& line) {
+  constexpr bool INV = DIR > 0;
+  constexpr int NS = fft_nstages(LEN);
+  if constexpr (S < NS) {
+    constexpr int E = fft_E(LEN), TPL = LEN / E;
+    constexpr int R = plan_radix(LEN, S, INV), P = plan_pstride(LEN, S, INV);
+    static_assert(E % R == 0, "radix must divide E");
+    for (int t = 0; t < TPL; ++t) {
+      cplx<T> r[E];
+      for (int j = 0; j < E; ++j) r[j] = v[t][j];
+      stage_compute<T, DIR, LEN, TPL, E, R, P, LEN>(r, t, tw);
+      for (int j = 0; j < E; ++j) v[t][j] = r[j];
+    }
+    if constexpr (S + 1 < NS) {
+      constexpr int NBT = E / R;
+      for (int t = 0; t < TPL; ++t)
+        for (int m = 0; m < NBT; ++m)
+          for (int q = 0; q < R; ++q) line[pad<T>(stockham_out(t + TPL * m, q, R, P))] = v[t][m + q * NBT];
+      for (int t = 0; t < TPL; ++t)
+        for (int j = 0; j < E; ++j) v[t][j] = line[pad<T>(t + TPL * j)];
+    }
+    run_stages<DIR, LEN, S + 1, T>(v, tw, line);
+  }
+}
+
+template <int LEN, class T>
+int check(double tol) {
+  constexpr int E = fft_E(LEN), TPL = LEN / E;
+  std::vector<cplx<T>> tw(LEN / 2);
+  for (int m = 0; m < LEN / 2; ++m) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * m / LEN;
+    tw[m] = cplx<T>{static_cast<T>(cosl(a)), static_cast<T>(sinl(a))};
+  }
+  std::vector<std::complex<double>> x(LEN);
+  unsigned s = 12345u + LEN;
+  for (auto& z : x) {
+    s = s * 1664525u + 1013904223u;
+    double a = (s >> 8) / 16777216.0 - 0.5;
+    s = s * 1664525u + 1013904223u;
+    double b = (s >> 8) / 16777216.0 - 0.5;
+    z = {a, b};
+  }
+  int bad = 0;
+  for (int dir : {-1, +1}) {
+    std::vector<std::vector<cplx<T>>> v(TPL, std::vector<cplx<T>>(E));
+    for (int t = 0; t < TPL; ++t)
+      for (int j = 0; j < E; ++j) {
+        const auto z = x[t + TPL * j];
+        v[t][j] = cplx<T>{static_cast<T>(z.real()), static_cast<T>(z.imag())};
+      }
+    std::vector<cplx<T>> line(padded_len<T>(LEN));
+    if (dir < 0) run_stages<-1, LEN, 0, T>(v, tw.data(), line);
+    else run_stages<+1, LEN, 0, T>(v, tw.data(), line);
+    double err = 0, mag = 0;
+    for (int k = 0; k < LEN; ++k) {
+      std::complex<long double> acc = 0;
+      for (int n = 0; n < LEN; ++n) {
+        const long double ang = dir * 2.0L * 3.141592653589793238462643383279502884L * ((long long)n * k % LEN) / LEN;
+        acc += std::complex<long double>(x[n].real(), x[n].imag()) * std::complex<long double>(cosl(ang), sinl(ang));
+      }
+      const cplx<T> g = v[k % TPL][k / TPL];
+      err = std::fmax(err, std::abs(std::complex<double>((double)g.x - (double)acc.real(), (double)g.y - (double)acc.imag())));
+      mag = std::fmax(mag, std::abs(std::complex<double>((double)acc.real(), (double)acc.imag())));
+    }
+    const bool ok = err <= tol * mag;
+    printf("LEN=%5d E=%2d stages=%d dir=%+d %s rel err %.3e\n", LEN, E, fft_nstages(LEN), dir,
+           sizeof(T) == 8 ? "f64" : "f32", err / mag);
+    if (!ok) { printf("  FAIL\n"); ++bad; }
+  }
+  return bad;
+}
+
+int main() {
+  int bad = 0;
+  bad += check<8, double>(1e-14);
+  bad += check<16, double>(1e-14);
+  bad += check<32, double>(1e-14);
+  bad += check<64, double>(1e-14);
+  bad += check<128, double>(1e-14);
+  bad += check<256, double>(1e-14);
+  bad += check<512, double>(1e-14);
+  bad += check<1024, double>(1e-14);
+  bad += check<2048, double>(1e-14);
+  bad += check<4096, double>(1e-14);
+  bad += check<1024, float>(2e-6);
+  bad += check<4096, float>(2e-6);
+  printf(bad ? "FAILED %d\n" : "ALL OK\n", bad);
+  return bad ? 1 : 0;
+}
