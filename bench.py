@@ -100,6 +100,19 @@ def run_form(pkg, form, c0, phi, a):
     return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass)
 
 
+def short_horizon_error(pkg, form, c0, phi, a, steps=1000):
+    """max|dc| of `form` against the FP64 GPU run after `steps` steps (t = 10 at
+    dt = 0.01, the horizon of the FP64 parity gate)."""
+    out = []
+    for f in ("fp64", form):
+        s = pkg.Session(a.n, 1.0, pkg.Parameters(), f)
+        s.set_fields(c0, phi)
+        s.run(a.dt, steps)
+        out.append(s.field())
+        s.close()
+    return float(np.max(np.abs(out[1] - out[0])))
+
+
 def cpu_baseline(c0, phi, a, c_gpu_steps=None):
     from oracle import core_shell_oracle as orc
     c0 = np.asarray(c0)
@@ -149,6 +162,7 @@ def main():
                  mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))))
         if ref is not None and f != "fp64":
             o["max_dc_vs_fp64"] = float(np.max(np.abs(r["c"] - ref["c"])))
+            o["max_dc_vs_fp64_1000_steps"] = short_horizon_error(pkg, f, c0, phi, a)
             o["rel_energy_err_vs_fp64"] = float(np.max(np.abs(r["hist"][:, 2] - ref["hist"][:, 2])
                                                       / np.abs(ref["hist"][:, 2])))
         out[f] = o

@@ -637,57 +637,59 @@ int csg_time_passes(csg_ctx* c, double dt, int64_t nsteps, double* ms) {
   if (nsteps < 1 || !ms) return fail(CSG_ERR_ARG, "nsteps >= 1 and ms required");
   HIPCHK(hipSetDevice(c->device));
   if (int rc = ensure_hist(c, static_cast<size_t>(nsteps) + 1)) return rc;
-  const size_t nn = static_cast<size_t>(c->n) * c->n;
-  // one real step first so A/Bb/partials hold the values of a live step
-  enqueue_p123(c, c->S);
-  enqueue_p4(c, c->S, dt, nullptr);
-  // the timed batches overwrite S/A/Bb repeatedly: keep copies to restore
-  void* keep = nullptr;
-  HIPCHK(hipMalloc(&keep, nn * (sizeof(double) + 2 * c->tsize)));
-  char* kp = static_cast<char*>(keep);
-  HIPCHK(hipMemcpyAsync(kp, c->S, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(kp + nn * sizeof(double), c->A, nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(kp + nn * (sizeof(double) + c->tsize), c->Bb, nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
-  hipEvent_t ev[6];
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   if (int rc = reset_ctr(c)) return rc;
-  // each pass launched nsteps times back to back, as in the captured loop
-  HIPCHK(hipEventRecord(ev[0], c->stream));
+  // loop iterations in the order of csg_run, with timing events (no system
+  // fence) recorded between the kernels on the context's stream
+  // all iterations are enqueued before the first wait, so no host submission
+  // gap falls inside an interval
+  std::vector<hipEvent_t> ev(static_cast<size_t>(4 * nsteps + 3));
+  int made = 0;
+  hipError_t e = hipSuccess;
+  for (auto& x : ev) {
+    e = hipEventCreateWithFlags(&x, hipEventDisableSystemFence);
+    if (e != hipSuccess) break;
+    ++made;
+  }
+  auto destroy = [&]() { for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]); };
+  if (e != hipSuccess) { destroy(); return fail(CSG_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)); }
+  size_t q = 0;
+  auto rec = [&]() { if (e == hipSuccess) e = hipEventRecord(ev[q++], c->stream); };
+  rec();
   for (int64_t s = 0; s < nsteps; ++s) {
     PassArgs a = base_args(c);
     a.in0 = c->S; a.out0 = c->A; a.part = part1(c);
     c->ops.p1(a, c->tw, c->stream);
-  }
-  HIPCHK(hipEventRecord(ev[1], c->stream));
-  for (int64_t s = 0; s < nsteps; ++s) {
-    PassArgs a = base_args(c);
+    rec();
+    a = base_args(c);
     a.in0 = c->S; a.in1 = c->A; a.out0 = c->A; a.out1 = c->Bb; a.part = part2(c);
     c->ops.p2(a, c->tw, c->stream);
-  }
-  HIPCHK(hipEventRecord(ev[2], c->stream));
-  for (int64_t s = 0; s < nsteps; ++s) {
-    PassArgs a = base_args(c);
+    rec();
+    a = base_args(c);
     a.in0 = c->A; a.in1 = c->Bb; a.out0 = c->A; a.out1 = c->Bb; a.part = part3(c);
     c->ops.p3(a, c->tw, c->stream);
+    rec();
+    enqueue_p4(c, c->S, dt, c->hist);
+    rec();
   }
-  HIPCHK(hipEventRecord(ev[3], c->stream));
-  for (int64_t s = 0; s < nsteps; ++s) enqueue_p4(c, c->S, dt, c->hist);
-  HIPCHK(hipEventRecord(ev[4], c->stream));
-  if (int rc = reset_ctr(c)) return rc;
-  for (int64_t s = 0; s < nsteps; ++s) enqueue_finalize(c, c->hist, dt);
-  HIPCHK(hipEventRecord(ev[5], c->stream));
-  HIPCHK(hipEventSynchronize(ev[5]));
-  for (int k = 0; k < 5; ++k) {
-    float t = 0.f;
-    HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-    ms[k] = t / static_cast<double>(nsteps);
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  HIPCHK(hipMemcpyAsync(c->S, kp, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->A, kp + nn * sizeof(double), nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->Bb, kp + nn * (sizeof(double) + c->tsize), nn * c->tsize, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipFree(keep));
+  // standalone history reduction (a run's last row), on a fresh counter
+  if (e == hipSuccess) e = hipMemsetAsync(c->ctr, 0, sizeof(long long), c->stream);
+  rec();
+  enqueue_finalize(c, c->hist, dt);
+  rec();
+  if (e == hipSuccess) e = hipEventSynchronize(ev[q - 1]);
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s)
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+      float t = 0.f;
+      e = hipEventElapsedTime(&t, ev[4 * s + k], ev[4 * s + k + 1]);
+      acc[k] += t;
+    }
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[q - 2], ev[q - 1]);
+  destroy();
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("csg_time_passes: ") + hipGetErrorString(e));
+  for (int k = 0; k < 4; ++k) ms[k] = acc[k] / static_cast<double>(nsteps);
+  ms[4] = t;
   return CSG_OK;
 }
 

@@ -116,11 +116,12 @@ int csg_initial_fields(int32_t n, double spacing, double radius, double width, i
 /* Julia's rand(MersenneTwister(seed), count) in column-major order, host only. */
 int csg_julia_rand(int64_t seed, int64_t count, double* out);
 
-/* Mean device time per launch of each pass kernel, from HIP events on the
- * context's stream around nsteps back-to-back launches of that kernel:
- * ms[0..3] = P1, P2, P3, P4 (P4 with the fused history-row reduction),
- * ms[4] = the standalone reduction kernel used for a run's last row.
- * Advances the resident field by one step (the timed launches are undone). */
+/* Mean device time per launch of each pass kernel over nsteps loop
+ * iterations, from HIP events (no system fence) recorded between the kernels
+ * of each iteration, launched in csg_run's order: ms[0..3] = P1, P2, P3,
+ * P4 (P4 with the fused history-row reduction), ms[4] = one launch of the
+ * standalone reduction kernel used for a run's last row.  Advances the
+ * resident field by nsteps steps. */
 int csg_time_passes(csg_ctx* ctx, double dt, int64_t nsteps, double* ms);
 
 /* Bytes of device memory held by the context. */

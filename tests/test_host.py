@@ -108,6 +108,20 @@ def test_writedlm_layout(pkg, tmp_path):
     assert p.read_text() == "0.5,1.0\n1.0e-5,-2.5\n"   # row i = first index
 
 
+def test_fft_engine_host_check(tmp_path):
+    """The register Stockham engine's stage code (plans, radix-16/8/4/2
+    butterflies, padded exchanges, twiddles) against a long-double DFT for
+    every line length 8..4096, FP64 and FP32 -- compiled for the host."""
+    import subprocess
+    exe = tmp_path / "fft_engine_check"
+    src = os.path.join(ROOT, "tests", "cpp", "fft_engine_check.hip")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-std=c++17", src, "-o", str(exe)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout
+
+
 def test_make_grid_matches_oracle(pkg):
     for n, sp in ((16, 1.0), (64, 0.5), (15, 2.0)):
         g = pkg.make_grid(n, sp)
