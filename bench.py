@@ -49,6 +49,9 @@ def _args():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--grad-steps", type=int, default=2000,
+                    help="steps of the 512^2 adjoint workload timed for gradient evaluations/s "
+                         "(0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 times), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -98,6 +101,30 @@ def run_form(pkg, form, c0, phi, a):
     ms_pass = s.time_passes(a.dt, 50)
     s.close()
     return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass)
+
+
+def gradient_rate(pkg, a):
+    """Gradient evaluations/s of the adjoint workload: 512^2, R=160, w=2,
+    a.grad_steps steps, sqrt-step checkpoints, loss 1/2||c(T)-c*||^2 + 1e-3 F."""
+    n, steps = 512, a.grad_steps
+    c0, phi = pkg.initial_fields(n, radius=160.0, width=2.0, seed=5749)
+    p0 = pkg.Parameters()
+    pt = pkg.Parameters(kappa=p0.kappa * 1.1, P=p0.P * 1.1, chi=p0.chi * 1.1)
+    s = pkg.Session(n, 1.0, pt, "fp64")
+    s.set_fields(c0, phi)
+    s.step(a.dt, steps)
+    tgt = s.field()
+    s.close()
+    pkg.adjoint.gradient(c0, phi, tgt, dt=a.dt, steps=steps, lam=1e-3)      # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = pkg.adjoint.gradient(c0, phi, tgt, dt=a.dt, steps=steps, lam=1e-3)
+    secs = (time.perf_counter() - t0) / reps
+    return {"workload": f"adjoint {n}x{n}, R=160, w=2, {steps} steps, checkpoints every "
+                        f"{max(1, round(steps ** 0.5))}", "gradient_evals_per_s": 1.0 / secs,
+            "seconds_per_gradient": secs, "loss": g.loss,
+            "forward_equivalent_site_updates_per_s": 3 * n * n * steps / secs}
 
 
 def short_horizon_error(pkg, form, c0, phi, a, steps=1000):
@@ -186,6 +213,8 @@ def main():
         "roofline": roof,
         "forms": out,
     }
+    if a.grad_steps:
+        line["gradient"] = gradient_rate(pkg, a)
     if not a.no_cpu:
         c_cpu, base = cpu_baseline(c0, phi, a)
         # the same number of GPU steps for a parity figure at full size

@@ -11,3 +11,4 @@ from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, S
                          initial_fields, make_grid, run_simulation, step, transport_rate)
 from ._capi import CsgError, lib  # noqa: F401
 from .writers import julia_float, writedlm  # noqa: F401
+from . import adjoint  # noqa: F401

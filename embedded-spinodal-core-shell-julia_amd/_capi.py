@@ -28,7 +28,7 @@ EXPORTS = (
     "csg_step", "csg_get_field", "csg_nonfinite_step", "csg_chemical_potential",
     "csg_transport_rate", "csg_energy", "csg_geometry", "csg_bulk_energy",
     "csg_bulk_derivative", "csg_initial_fields", "csg_julia_rand", "csg_time_passes",
-    "csg_device_bytes", "csg_last_error",
+    "csg_device_bytes", "csg_last_error", "csg_adjoint_forward", "csg_adjoint_backward",
 )
 
 
@@ -76,6 +76,8 @@ def lib():
         "csg_julia_rand": (ctypes.c_int, [i64, i64, _D]),
         "csg_time_passes": (ctypes.c_int, [_VP, dbl, i64, _D]),
         "csg_device_bytes": (i64, [_VP]),
+        "csg_adjoint_forward": (ctypes.c_int, [_VP, dbl, i64, i64]),
+        "csg_adjoint_backward": (ctypes.c_int, [_VP, _D, _D, _D]),
         "csg_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -7,10 +7,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/csg.h"
-#include "csg_kernels.hpp"
+#include "csg_adjoint.hpp"
 
 using namespace csg;
 
@@ -39,6 +40,11 @@ struct Ops {
   LaunchFn inv_bulk;   // bulk-precision line array -> real rows (double)
   LaunchFn p1, p2, p3, p4, mu, rate;
   int n_row1, n_col, n_row3;  // workgroups of P1, P2, P3 (partials count)
+  // adjoint passes (FP64 contexts only)
+  void (*adj1)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
+  void (*adj3)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
+  void (*adj4)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
+  int n_adj_col = 0, n_adj_row = 0;
 };
 
 template <class CT, class TIN, class TOUT, int N, int MODE>
@@ -53,6 +59,22 @@ void launch_col(const PassArgs& a, const void* tw, hipStream_t s) {
   using Cfg = ColCfg<CT, TS, N, MODE>;
   hipLaunchKernelGGL((col_pass<CT, TS, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
                      static_cast<const cplx<CT>*>(tw));
+}
+
+template <int N>
+void launch_adj1(const AdjArgs& a, const void* tw, hipStream_t s) {
+  hipLaunchKernelGGL(adj_col1<N>, dim3(AdjColCfg<N>::GRID), dim3(AdjColCfg<N>::NT), 0, s, a,
+                     static_cast<const cplx<double>*>(tw));
+}
+template <int N>
+void launch_adj3(const AdjArgs& a, const void* tw, hipStream_t s) {
+  hipLaunchKernelGGL(adj_col3<N>, dim3(AdjColCfg<N>::GRID), dim3(AdjColCfg<N>::NT), 0, s, a,
+                     static_cast<const cplx<double>*>(tw));
+}
+template <int N>
+void launch_adj4(const AdjArgs& a, const void* tw, hipStream_t s) {
+  hipLaunchKernelGGL(adj_row4<N>, dim3(AdjRowCfg<N>::GRID), dim3(AdjRowCfg<N>::NT), 0, s, a,
+                     static_cast<const cplx<double>*>(tw));
 }
 
 // T: bulk (compute) type; the state is always double.
@@ -71,6 +93,13 @@ Ops make_ops() {
   o.n_row1 = RowCfg<T, double, T, N, ROW_P1>::GRID;
   o.n_row3 = RowCfg<T, T, T, N, ROW_P3>::GRID;
   o.n_col = ColCfg<T, double, N, COL_P2>::GRID;
+  if constexpr (std::is_same<T, double>::value) {
+    o.adj1 = launch_adj1<N>;
+    o.adj3 = launch_adj3<N>;
+    o.adj4 = launch_adj4<N>;
+    o.n_adj_col = AdjColCfg<N>::GRID;
+    o.n_adj_row = AdjRowCfg<N>::GRID;
+  }
   return o;
 }
 
@@ -124,6 +153,14 @@ struct csg_ctx {
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
   size_t bytes = 0;
+  // adjoint: checkpoints of the forward run, segment states, adjoint field
+  std::vector<void*> ckpt;
+  std::vector<void*> seg;
+  void* G = nullptr;
+  void* Xr = nullptr;
+  double* adj_acc = nullptr;
+  int64_t adj_steps = -1, adj_every = 0;
+  double adj_dt = 0.0;
 };
 
 namespace {
@@ -364,8 +401,12 @@ int csg_destroy(csg_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   void* bufs[] = {c->S, c->S2, c->A, c->Bb, c->phi, c->real, c->part, c->ctr, c->hist,
-                  c->scratch_row, c->rate, c->tw64};
+                  c->scratch_row, c->rate, c->tw64, c->G, c->Xr, c->adj_acc};
   for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (void* b : c->ckpt)
+    if (b) (void)hipFree(b);
+  for (void* b : c->seg)
     if (b) (void)hipFree(b);
   if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -690,6 +731,124 @@ int csg_time_passes(csg_ctx* c, double dt, int64_t nsteps, double* ms) {
   if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("csg_time_passes: ") + hipGetErrorString(e));
   for (int k = 0; k < 4; ++k) ms[k] = acc[k] / static_cast<double>(nsteps);
   ms[4] = t;
+  return CSG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint: checkpointed forward run and backward sweep (FP64 contexts).
+namespace {
+
+int ensure_bufs(csg_ctx* c, std::vector<void*>& v, size_t count, size_t bytes) {
+  while (v.size() < count) {
+    void* p = nullptr;
+    if (int rc = dmalloc(c, &p, bytes)) return rc;
+    v.push_back(p);
+  }
+  return CSG_OK;
+}
+
+void enqueue_plain_step(csg_ctx* c, void* S, double dt) {
+  enqueue_p123(c, S);
+  enqueue_p4(c, S, dt, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_adjoint_forward(csg_ctx* c, double dt, int64_t steps, int64_t ckpt_every) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (!c->ops.adj1) return fail(CSG_ERR_UNSUPPORTED, "the adjoint needs an FP64 context");
+  if (steps < 1 || ckpt_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/ckpt_every/dt");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
+  const size_t nck = static_cast<size_t>(steps / ckpt_every) + 1;
+  if (int rc = ensure_bufs(c, c->ckpt, nck, bytes)) return rc;
+  HIPCHK(hipMemcpyAsync(c->ckpt[0], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  for (int64_t s = 0; s < steps; ++s) {
+    enqueue_plain_step(c, c->S, dt);
+    if ((s + 1) % ckpt_every == 0)
+      HIPCHK(hipMemcpyAsync(c->ckpt[(s + 1) / ckpt_every], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->adj_steps = steps;
+  c->adj_every = ckpt_every;
+  c->adj_dt = dt;
+  return CSG_OK;
+}
+
+int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, double* dL_dparams) {
+  if (int rc = check_ctx(c)) return rc;
+  if (c->adj_steps < 1) return fail(CSG_ERR_STATE, "no checkpointed forward run (csg_adjoint_forward)");
+  if (!dL_dcT) return fail(CSG_ERR_ARG, "dL_dcT is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t T = c->adj_steps, K = c->adj_every;
+  const double dt = c->adj_dt;
+  const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
+  if (!c->G && dmalloc(c, &c->G, bytes)) return CSG_ERR_HIP;
+  if (!c->Xr && dmalloc(c, &c->Xr, bytes)) return CSG_ERR_HIP;
+  const int na = c->ops.n_adj_col, nr = c->ops.n_adj_row;
+  const size_t nacc = 2 * static_cast<size_t>(na) + nr;
+  if (!c->adj_acc && dmalloc(c, reinterpret_cast<void**>(&c->adj_acc), nacc * sizeof(double))) return CSG_ERR_HIP;
+  if (int rc = ensure_bufs(c, c->seg, static_cast<size_t>(K) + 1, bytes)) return rc;
+  HIPCHK(hipMemsetAsync(c->adj_acc, 0, nacc * sizeof(double), c->stream));
+  // gbar_T into the state's packed line layout
+  if (int rc = upload_real(c, dL_dcT)) return rc;
+  if (int rc = set_state_from_real(c, c->G)) return rc;
+  AdjArgs a{};
+  a.G = static_cast<cplx<double>*>(c->G);
+  a.X0 = static_cast<cplx<double>*>(c->A);
+  a.X1 = static_cast<cplx<double>*>(c->Bb);
+  a.Xr = static_cast<cplx<double>*>(c->Xr);
+  a.phi = static_cast<const double*>(c->phi);
+  a.dt = dt;
+  a.p = c->dp;
+  const int64_t nseg = (T + K - 1) / K;
+  for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+    const int64_t s0 = sg * K, s1 = std::min<int64_t>(s0 + K, T), m = s1 - s0;
+    // recompute the segment's states from its checkpoint
+    HIPCHK(hipMemcpyAsync(c->seg[0], c->ckpt[sg], bytes, hipMemcpyDeviceToDevice, c->stream));
+    for (int64_t i = 0; i < m; ++i) {
+      HIPCHK(hipMemcpyAsync(c->seg[i + 1], c->seg[i], bytes, hipMemcpyDeviceToDevice, c->stream));
+      enqueue_plain_step(c, c->seg[i + 1], dt);
+    }
+    for (int64_t i = m - 1; i >= 0; --i) {
+      a.S0 = static_cast<const cplx<double>*>(c->seg[i]);
+      a.S1 = static_cast<const cplx<double>*>(c->seg[i + 1]);
+      a.acc = c->adj_acc;
+      c->ops.adj1(a, c->tw, c->stream);
+      PassArgs pa = base_args(c);
+      pa.in0 = c->A; pa.in1 = c->Bb; pa.out0 = c->A; pa.out1 = c->Bb; pa.part = part3(c);
+      c->ops.p3(pa, c->tw, c->stream);
+      a.acc = c->adj_acc + na;
+      c->ops.adj3(a, c->tw, c->stream);
+      a.acc = c->adj_acc + 2 * na;
+      c->ops.adj4(a, c->tw, c->stream);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<double> acc(nacc);
+  HIPCHK(hipMemcpyAsync(acc.data(), c->adj_acc, nacc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (dL_dc0) {
+    PassArgs pa = base_args(c);
+    pa.in0 = c->G;
+    pa.out0 = c->real;
+    c->ops.inv_state(pa, c->tw64, c->stream);
+    HIPCHK(hipMemcpyAsync(dL_dc0, c->real, bytes, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dL_dparams) {
+    double kap = 0.0, srg = 0.0;
+    for (int i = 0; i < 2 * na; ++i) kap += acc[i];
+    for (int i = 0; i < nr; ++i) srg += acc[2 * na + i];
+    dL_dparams[0] = kap;
+    dL_dparams[1] = -c->p.chi * srg;
+    dL_dparams[2] = -c->p.P * srg;
+  }
   return CSG_OK;
 }
 

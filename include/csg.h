@@ -124,6 +124,22 @@ int csg_julia_rand(int64_t seed, int64_t count, double* out);
  * resident field by nsteps steps. */
 int csg_time_passes(csg_ctx* ctx, double dt, int64_t nsteps, double* ms);
 
+/* Adjoint of the time-step loop, FP64 contexts only.  No counterpart in the
+ * reference (which has no derivatives); it differentiates step
+ * (core_shell.jl:72-76) iterated from the resident field.
+ *
+ * csg_adjoint_forward: advance the resident field by `steps` steps of `dt`,
+ * keeping checkpoints every `ckpt_every` steps (sqrt(steps) is the usual
+ * choice); the field c(T) stays resident. */
+int csg_adjoint_forward(csg_ctx* ctx, double dt, int64_t steps, int64_t ckpt_every);
+
+/* csg_adjoint_backward: given dL/dc(T) (host, column-major N x N), return
+ * dL/dc(0) (may be NULL) and dL_dparams[0..2] = the gradient with respect to
+ * (kappa, P, chi) through the dynamics (may be NULL); the loss's own direct
+ * dependence on the parameters is the caller's.  Recomputes each segment's
+ * states from its checkpoint; the resident field is not changed. */
+int csg_adjoint_backward(csg_ctx* ctx, const double* dL_dcT, double* dL_dc0, double* dL_dparams);
+
 /* Bytes of device memory held by the context. */
 int64_t csg_device_bytes(const csg_ctx* ctx);
 

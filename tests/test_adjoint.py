@@ -1,0 +1,153 @@
+"""Adjoint of the time loop.
+
+CPU: the oracle's adjoint (oracle/adjoint_oracle.py) against central finite
+differences of the oracle's own forward loss.
+GPU: the HIP adjoint (csg_adjoint_forward / csg_adjoint_backward) against the
+oracle adjoint (same inputs, tight tolerance) and against central finite
+differences of the GPU forward loss, including the 512^2, 2000-step,
+45-step-checkpoint workload on 32 random directions.
+Loss: L = 1/2 ||c(T) - c*||^2 + lam F(c(T)), c* = forward run with kappa, P,
+chi raised 10 %.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import adjoint_oracle as ad
+from oracle import core_shell_oracle as o
+
+P = o.Parameters()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _setup_oracle(n, radius, steps, dt, lam):
+    c0, phi = o.initial_fields(n, radius=radius, width=2.0)
+    pt = dataclasses.replace(P, kappa=P.kappa * 1.1, P=P.P * 1.1, chi=P.chi * 1.1)
+    tgt = ad.forward(c0, o.geometry(phi, pt), o.make_grid(n, 1.0), pt, dt, steps)[-1]
+    return c0, phi, ad.Loss(tgt, lam=lam)
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    n, steps, dt, eps = 32, 40, 0.01, 1e-6
+    c0, phi, loss = _setup_oracle(n, 8.0, steps, dt, 0.1)
+    L, g, dth = ad.gradient(c0, phi, P, dt, steps, loss)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        v = rng.standard_normal(c0.shape)
+        fd = (ad.loss_of(c0 + eps * v, phi, P, dt, steps, loss)
+              - ad.loss_of(c0 - eps * v, phi, P, dt, steps, loss)) / (2 * eps)
+        assert abs(fd - np.sum(g * v)) <= 1e-6 * abs(fd)
+    for i, name in enumerate(("kappa", "P", "chi")):
+        pp = dataclasses.replace(P, **{name: getattr(P, name) + eps})
+        pm = dataclasses.replace(P, **{name: getattr(P, name) - eps})
+        fd = (ad.loss_of(c0, phi, pp, dt, steps, loss) - ad.loss_of(c0, phi, pm, dt, steps, loss)) / (2 * eps)
+        assert abs(fd - dth[i]) <= 1e-6 * abs(fd), name
+
+
+def test_oracle_adjoint_planted_sign_error_fails():
+    """Flipping the sign of the stabilisation term in the VJP breaks FD agreement."""
+    n, steps, dt, eps = 16, 10, 0.05, 1e-6
+    c0, phi, loss = _setup_oracle(n, 4.0, steps, dt, 0.0)
+    grid, shape = o.make_grid(n, 1.0), o.geometry(phi, P)
+    cs = ad.forward(c0, shape, grid, P, dt, steps)
+    gbar = cs[-1] - loss.target
+    bad = dataclasses.replace(P, Q=-P.Q)
+    for s in range(steps - 1, -1, -1):
+        gbar, _ = ad.step_vjp(cs[s], gbar, shape, grid, bad, dt)
+    v = np.random.default_rng(1).standard_normal(c0.shape)
+    fd = (ad.loss_of(c0 + eps * v, phi, P, dt, steps, loss) - ad.loss_of(c0 - eps * v, phi, P, dt, steps, loss)) / (2 * eps)
+    assert abs(fd - np.sum(gbar * v)) > 1e-4 * abs(fd)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_oracle_adjoint(pkg):
+    say("adjoint: GPU vs oracle, 64^2, 200 steps")
+    n, steps, dt = 64, 200, 0.01
+    c0, phi, loss = _setup_oracle(n, 16.0, steps, dt, 0.1)
+    L, g, dth = ad.gradient(c0, phi, P, dt, steps, loss)
+    r = pkg.adjoint.gradient(c0, phi, loss.target, dt=dt, steps=steps, lam=0.1, ckpt_every=14)
+    assert abs(r.loss / L - 1) < 1e-12
+    assert np.max(np.abs(r.dc0 - g)) <= 1e-10 * np.max(np.abs(g))
+    for got, ref in zip((r.dkappa, r.dP, r.dchi), dth):
+        assert abs(got - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ckpt", [1, 7, 50])
+def test_gpu_adjoint_checkpoint_interval_invariant(pkg, ckpt):
+    say(f"adjoint: checkpoint interval {ckpt}")
+    n, steps, dt = 32, 50, 0.01
+    c0, phi, loss = _setup_oracle(n, 8.0, steps, dt, 0.0)
+    ref = pkg.adjoint.gradient(c0, phi, loss.target, dt=dt, steps=steps, ckpt_every=50)
+    r = pkg.adjoint.gradient(c0, phi, loss.target, dt=dt, steps=steps, ckpt_every=ckpt)
+    assert np.array_equal(r.dc0, ref.dc0)
+    assert (r.dkappa, r.dP, r.dchi) == (ref.dkappa, ref.dP, ref.dchi)
+
+
+def _fd_check(pkg, n, radius, steps, dt, lam, ckpt, ndir, seed=0):
+    """Relative errors of the adjoint against central FD: directions (unit
+    2-norm) at eps = 1e-5 and 1e-6, parameters at eps = 1e-6."""
+    c0, phi = pkg.initial_fields(n, radius=radius, width=2.0)
+    c0, phi = np.asarray(c0), np.asarray(phi)
+    pt = pkg.Parameters(kappa=P.kappa * 1.1, P=P.P * 1.1, chi=P.chi * 1.1)
+    s = pkg.Session(n, 1.0, pt, "fp64")
+    s.set_fields(c0, phi)
+    s.step(dt, steps)
+    tgt = s.field()
+    s.close()
+    r = pkg.adjoint.gradient(c0, phi, tgt, dt=dt, steps=steps, lam=lam, ckpt_every=ckpt)
+    rng = np.random.default_rng(seed)
+    errs = {1e-5: [], 1e-6: []}
+    for _ in range(ndir):
+        # unit 2-norm directions keep eps * v in the linear regime of the map
+        v = rng.standard_normal(c0.shape)
+        v /= np.linalg.norm(v)
+        an = np.sum(r.dc0 * v)
+        for eps in errs:
+            fd = (pkg.adjoint.loss(c0 + eps * v, phi, tgt, dt=dt, steps=steps, lam=lam)
+                  - pkg.adjoint.loss(c0 - eps * v, phi, tgt, dt=dt, steps=steps, lam=lam)) / (2 * eps)
+            errs[eps].append(abs(fd - an) / abs(fd))
+    perr = []
+    eps = 1e-6
+    for name, got in (("kappa", r.dkappa), ("P", r.dP), ("chi", r.dchi)):
+        pp = pkg.Parameters(**{name: getattr(P, name) + eps})
+        pm = pkg.Parameters(**{name: getattr(P, name) - eps})
+        fd = (pkg.adjoint.loss(c0, phi, tgt, dt=dt, steps=steps, lam=lam, p=pp)
+              - pkg.adjoint.loss(c0, phi, tgt, dt=dt, steps=steps, lam=lam, p=pm)) / (2 * eps)
+        perr.append(abs(fd - got) / abs(fd))
+    say(f"adjoint FD n={n} steps={steps}: directions eps=1e-5 max {max(errs[1e-5]):.2e}, "
+        f"eps=1e-6 median {np.median(errs[1e-6]):.2e} max {max(errs[1e-6]):.2e}; params {max(perr):.2e}")
+    return errs, perr
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_finite_differences_128(pkg):
+    errs, perr = _fd_check(pkg, 128, 40.0, 500, 0.01, 1e-3, 22, 8)
+    assert max(errs[1e-5]) <= 1e-6 and max(errs[1e-6]) <= 1e-6 and max(perr) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_512_workload(pkg):
+    """512^2, R=160, 2000 steps, sqrt(N)=45-step checkpoints, 32 directions.
+    At eps = 1e-6 the FD of this 2000-step loss is roundoff-limited (FD error
+    grows again below eps ~1e-5), so its bound there is on the median."""
+    errs, perr = _fd_check(pkg, 512, 160.0, 2000, 0.01, 1e-3, 45, 32)
+    assert max(errs[1e-5]) <= 1e-6
+    assert np.median(errs[1e-6]) <= 1e-6 and max(errs[1e-6]) <= 1e-5
+    assert max(perr) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_mixed_context_rejects_adjoint(pkg):
+    from csg_amd import _capi
+    s = pkg.Session(32, 1.0, pkg.Parameters(), "mixed")
+    c0, phi = pkg.initial_fields(32, radius=8.0)
+    s.set_fields(c0, phi)
+    assert pkg.lib().csg_adjoint_forward(s.handle, 0.01, 10, 3) == _capi.CSG_ERR_UNSUPPORTED
+    s.close()
