@@ -25,9 +25,14 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wno-unused-command-line-argument"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-# One translation unit per (precision, size group) so builds run in parallel.
+# The kernel instantiations are split by (precision, size family) so the
+# translation units compile in parallel.
 UNITS = [
     ("csg_capi.hip", []),
+    ("csg_ops_fp64_pow2.hip", []),
+    ("csg_ops_fp64_pow3.hip", []),
+    ("csg_ops_mixed_pow2.hip", []),
+    ("csg_ops_mixed_pow3.hip", []),
 ]
 
 

@@ -58,7 +58,7 @@ __device__ __forceinline__ void partner_of(const cplx<double> (&src)[E], cplx<do
   for (int j = 0; j < E; ++j) line[pad<double>(t + TPL * j)] = src[j];
   lds_sync<WAVE>();
 #pragma unroll
-  for (int j = 0; j < E; ++j) dst[j] = line[pad<double>((N - (t + TPL * j)) & (N - 1))];
+  for (int j = 0; j < E; ++j) dst[j] = line[pad<double>(neg_index<N>(t + TPL * j))];
   lds_sync<WAVE>();
 }
 
@@ -188,7 +188,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     const cplx<double> ch = v[2][j];
     cplx<double> r, lap;
     if (kx == 0) {
-      const int kn = (N - ky) & (N - 1);
+      const int kn = neg_index<N>(ky);
       r = mul_i(sy * v[1][j]);
       const cplx<double> rn = mul_i(dkval<N>(kn, inv_nsp) * jyn[j]);
       const double k2a = ky2, k2b = kN2 + ky2;

@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/csg.h"
-#include "csg_adjoint.hpp"
+#include "csg_ops.hpp"
 
 using namespace csg;
 
@@ -31,92 +31,10 @@ int fail(int code, const std::string& msg) {
       return fail(CSG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
-using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
-
-// Kernel table for one (N, precision).
-struct Ops {
-  LaunchFn fwd;        // real rows (double) -> state S (line layout, double)
-  LaunchFn inv_state;  // state S -> real rows (double)
-  LaunchFn inv_bulk;   // bulk-precision line array -> real rows (double)
-  LaunchFn p1, p2, p3, p4, mu, rate;
-  int n_row1, n_col, n_row3;  // workgroups of P1, P2, P3 (partials count)
-  // adjoint passes (FP64 contexts only)
-  void (*adj1)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
-  void (*adj3)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
-  void (*adj4)(const AdjArgs&, const void* tw, hipStream_t) = nullptr;
-  int n_adj_col = 0, n_adj_row = 0;
-};
-
-template <class CT, class TIN, class TOUT, int N, int MODE>
-void launch_row(const PassArgs& a, const void* tw, hipStream_t s) {
-  using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
-  hipLaunchKernelGGL((row_pass<CT, TIN, TOUT, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
-                     static_cast<const cplx<CT>*>(tw));
-}
-
-template <class CT, class TS, int N, int MODE>
-void launch_col(const PassArgs& a, const void* tw, hipStream_t s) {
-  using Cfg = ColCfg<CT, TS, N, MODE>;
-  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
-                     static_cast<const cplx<CT>*>(tw));
-}
-
-template <int N>
-void launch_adj1(const AdjArgs& a, const void* tw, hipStream_t s) {
-  hipLaunchKernelGGL(adj_col1<N>, dim3(AdjColCfg<N>::GRID), dim3(AdjColCfg<N>::NT), 0, s, a,
-                     static_cast<const cplx<double>*>(tw));
-}
-template <int N>
-void launch_adj3(const AdjArgs& a, const void* tw, hipStream_t s) {
-  hipLaunchKernelGGL(adj_col3<N>, dim3(AdjColCfg<N>::GRID), dim3(AdjColCfg<N>::NT), 0, s, a,
-                     static_cast<const cplx<double>*>(tw));
-}
-template <int N>
-void launch_adj4(const AdjArgs& a, const void* tw, hipStream_t s) {
-  hipLaunchKernelGGL(adj_row4<N>, dim3(AdjRowCfg<N>::GRID), dim3(AdjRowCfg<N>::NT), 0, s, a,
-                     static_cast<const cplx<double>*>(tw));
-}
-
-// T: bulk (compute) type; the state is always double.
-template <int N, class T>
-Ops make_ops() {
-  Ops o;
-  o.fwd = launch_row<double, double, double, N, ROW_FWD>;
-  o.inv_state = launch_row<double, double, double, N, ROW_INV>;
-  o.inv_bulk = launch_row<T, T, double, N, ROW_INV>;
-  o.p1 = launch_row<T, double, T, N, ROW_P1>;
-  o.p3 = launch_row<T, T, T, N, ROW_P3>;
-  o.p2 = launch_col<T, double, N, COL_P2>;
-  o.mu = launch_col<T, double, N, COL_MU>;
-  o.p4 = launch_col<T, double, N, COL_P4>;
-  o.rate = launch_col<T, double, N, COL_RATE>;
-  o.n_row1 = RowCfg<T, double, T, N, ROW_P1>::GRID;
-  o.n_row3 = RowCfg<T, T, T, N, ROW_P3>::GRID;
-  o.n_col = ColCfg<T, double, N, COL_P2>::GRID;
-  if constexpr (std::is_same<T, double>::value) {
-    o.adj1 = launch_adj1<N>;
-    o.adj3 = launch_adj3<N>;
-    o.adj4 = launch_adj4<N>;
-    o.n_adj_col = AdjColCfg<N>::GRID;
-    o.n_adj_row = AdjRowCfg<N>::GRID;
-  }
-  return o;
-}
-
 template <class T>
 bool ops_for(int n, Ops* o) {
-  switch (n) {
-    case 16: *o = make_ops<16, T>(); return true;
-    case 32: *o = make_ops<32, T>(); return true;
-    case 64: *o = make_ops<64, T>(); return true;
-    case 128: *o = make_ops<128, T>(); return true;
-    case 256: *o = make_ops<256, T>(); return true;
-    case 512: *o = make_ops<512, T>(); return true;
-    case 1024: *o = make_ops<1024, T>(); return true;
-    case 2048: *o = make_ops<2048, T>(); return true;
-    case 4096: *o = make_ops<4096, T>(); return true;
-    default: return false;
-  }
+  if constexpr (std::is_same<T, double>::value) return ops_fp64_pow2(n, o) || ops_fp64_pow3(n, o);
+  else return ops_mixed_pow2(n, o) || ops_mixed_pow3(n, o);
 }
 
 constexpr int FIN_NT = 256;
@@ -347,7 +265,7 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
   if (!ok) {
     delete c;
     return fail(CSG_ERR_UNSUPPORTED, "grid size " + std::to_string(n) +
-                                         " not supported (N = 2^m, 16 <= N <= 4096)");
+                                         " not supported (N = 2^m with 16 <= N <= 4096, or N = 3*2^m with 48 <= N <= 768)");
   }
   c->dp = DevParams{c->p.A, c->p.B, c->p.chi, c->p.P, c->p.ca, c->p.cb, c->p.cc,
                     c->p.kappa, c->p.mobility, c->p.Q, spacing, (1.0 / spacing) / n};

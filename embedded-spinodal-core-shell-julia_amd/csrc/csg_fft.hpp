@@ -129,8 +129,21 @@ template <int DIR, int ST, class T> CSG_HD void dft16(cplx<T>* x) {
     for (int k1 = 0; k1 < 4; ++k1) x[(k2 + 4 * k1) * ST] = a[4 * k2 + k1];
 }
 
+// radix 3: W_3 = exp(DIR * 2 pi i / 3) = -1/2 + DIR i sqrt(3)/2
+template <int DIR, int ST, class T> CSG_HD void dft3(cplx<T>* x) {
+  const T s = static_cast<T>(DIR * 0.86602540378443864676372317075293618347);  // DIR sqrt(3)/2
+  const cplx<T> a = x[0], b = x[ST], c = x[2 * ST];
+  const cplx<T> t = b + c;
+  const cplx<T> m = a - static_cast<T>(0.5) * t;      // a + (b + c) Re W_3
+  const cplx<T> d = s * mul_i(b - c);                 // (b - c) i Im W_3
+  x[0] = a + t;
+  x[ST] = m + d;
+  x[2 * ST] = m - d;
+}
+
 template <int R, int DIR, int ST, class T> CSG_HD void dft(cplx<T>* x) {
   if constexpr (R == 2) dft2<DIR, ST>(x);
+  else if constexpr (R == 3) dft3<DIR, ST>(x);
   else if constexpr (R == 4) dft4<DIR, ST>(x);
   else if constexpr (R == 8) dft8<DIR, ST>(x);
   else dft16<DIR, ST>(x);
@@ -142,17 +155,25 @@ template <int R, int DIR, int ST, class T> CSG_HD void dft(cplx<T>* x) {
 #ifndef CSG_FFT_E_BIG
 #define CSG_FFT_E_BIG 8
 #endif
+// Lengths 3 * 2^m (384^2 grids) use 24 elements per thread and one radix-3
+// stage first.
 constexpr int fft_E(int len) {
-  return len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : (len >= 1024 ? CSG_FFT_E_BIG : 8));
+  return (len % 3 == 0) ? (len < 24 ? len : 24)
+         : len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : (len >= 1024 ? CSG_FFT_E_BIG : 8));
 }
-// Radix of forward stage s: greedy largest radix <= E dividing what remains.
+// Radix of forward stage s: 3 while 3 divides what remains, then the largest
+// power of two <= 16 dividing both E and what remains.
 constexpr int fft_radix(int len, int s) {
   const int e = fft_E(len);
   int rem = len;
   int r = 1;
   for (int k = 0; k <= s; ++k) {
-    r = 16;
-    while (r > e || rem % r != 0) r >>= 1;
+    if (rem % 3 == 0) {
+      r = 3;
+    } else {
+      r = 16;
+      while (r > e || rem % r != 0 || e % r != 0) r >>= 1;
+    }
     rem /= r;
   }
   return r;
@@ -181,6 +202,9 @@ template <class T> constexpr int pad_shift() { return sizeof(T) == 8 ? 3 : 4; }
 template <class T> CSG_HD int pad(int e) { return e + (e >> pad_shift<T>()); }
 template <class T> constexpr int padded_len(int len) { return len + (len >> pad_shift<T>()) + 1; }
 
+// Index of -k on a periodic line of length N.
+template <int N> CSG_HD int neg_index(int k) { return k == 0 ? 0 : N - k; }
+
 // Stockham output position of value q of butterfly i at stride product P.
 CSG_HD int stockham_out(int i, int q, int R, int P) {
   const int k = i % P;
@@ -192,8 +216,10 @@ CSG_HD int stockham_out(int i, int q, int R, int P) {
 // W_NTAB^e (0 <= e < NTAB) for direction DIR from the half table.
 template <int DIR, int NTAB, class T>
 CSG_HD cplx<T> tw_at(const cplx<T>* tw, int e) {
-  cplx<T> w = tw[e & (NTAB / 2 - 1)];
-  if (e & (NTAB / 2)) w = cplx<T>{-w.x, -w.y};
+  constexpr int H = NTAB / 2;
+  const bool hi = e >= H;
+  cplx<T> w = tw[hi ? e - H : e];
+  if (hi) w = cplx<T>{-w.x, -w.y};
   if constexpr (DIR > 0) w.y = -w.y;
   return w;
 }
@@ -248,7 +274,9 @@ __device__ __forceinline__ void stage_exchange(cplx<T> (&v)[E], int t, cplx<T>* 
   lds_sync<WAVE>();
 #pragma unroll
   for (int j = 0; j < E; ++j) v[j] = line[pad<T>(t + TPL * j)];
-  if constexpr (!WAVE) __syncthreads();
+  // the next exchange overwrites what other lanes read here: without this
+  // ordering point the compiler may hoist its stores above these loads
+  lds_sync<WAVE>();
 }
 
 template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE>
@@ -288,9 +316,12 @@ template <int OP> __device__ __forceinline__ double red_combine(double a, double
   else return (a != a || b != b) ? a + b : fmax(a, b);
 }
 
-template <int OP> __device__ __forceinline__ double wave_reduce(double v) {
+// Reduce over the W = min(NT, 64) active lanes of a wavefront (NT a power of
+// two): partners lane ^ o stay below W, so lanes of a workgroup narrower than
+// a wavefront never read from lanes that do not run.
+template <int OP, int W> __device__ __forceinline__ double wave_reduce(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = red_combine<OP>(v, __shfl_xor(v, o, 64));
+  for (int o = W / 2; o > 0; o >>= 1) v = red_combine<OP>(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -299,12 +330,14 @@ template <int OP> __device__ __forceinline__ double wave_reduce(double v) {
 template <int NT, int K>
 __device__ __forceinline__ void block_reduce(double (&v)[K], const int (&ops)[K], double* scratch) {
   constexpr int NW = (NT + 63) / 64;
+  constexpr int W = NT < 64 ? NT : 64;
+  static_assert((W & (W - 1)) == 0 && (NT <= 64 || NT % 64 == 0), "whole wavefronts or one power-of-two part");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
-    if (ops[q] == RED_SUM) v[q] = wave_reduce<RED_SUM>(v[q]);
-    else if (ops[q] == RED_MIN) v[q] = wave_reduce<RED_MIN>(v[q]);
-    else v[q] = wave_reduce<RED_MAX>(v[q]);
+    if (ops[q] == RED_SUM) v[q] = wave_reduce<RED_SUM, W>(v[q]);
+    else if (ops[q] == RED_MIN) v[q] = wave_reduce<RED_MIN, W>(v[q]);
+    else v[q] = wave_reduce<RED_MAX, W>(v[q]);
   }
   __syncthreads();
   if (lane == 0) {

@@ -148,8 +148,8 @@ constexpr int row_rows() {
   constexpr int L = N / 2, TPL = L / fft_E(L);
   const int lp = NARR * padded_len<CT>(L) * 2 * (int)sizeof(CT);
   int b = 1;
-  while (b < 8 && b * 2 <= N && (b * 2) * lp <= CSG_ROW_LDS && (b * 2) * TPL <= 1024) b *= 2;
-  while (b * TPL < 64 && b * 2 <= N) b *= 2;
+  while (b < 8 && N % (b * 2) == 0 && (b * 2) * lp <= CSG_ROW_LDS && (b * 2) * TPL <= 1024) b *= 2;
+  while (b * TPL < 64 && N % (b * 2) == 0) b *= 2;
   return b;
 }
 
@@ -382,8 +382,8 @@ constexpr int col_lines() {
   if (TPL > 64) return 1;   // lines spanning wavefronts: one per workgroup
   const int lp = padded_len<CT>(N) * 2 * (int)sizeof(CT);
   int c = 1;
-  while (c * 2 <= N / 2 && (c * 2) * lp <= 40 * 1024 && (c * 2) * TPL <= 256) c *= 2;
-  while (c * TPL < 64 && c * 2 <= N / 2) c *= 2;
+  while ((N / 2) % (c * 2) == 0 && (c * 2) * lp <= 40 * 1024 && (c * 2) * TPL <= 256) c *= 2;
+  while (c * TPL < 64 && (N / 2) % (c * 2) == 0) c *= 2;
   return c;
 }
 
@@ -449,7 +449,7 @@ col_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
     for (int j = 0; j < E; ++j) line[pad<CT>(t + TPL * j)] = v[PA][j];
     lds_sync<WAVE>();
 #pragma unroll
-    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT>((N - (t + TPL * j)) & (N - 1))];
+    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT>(neg_index<N>(t + TPL * j))];
     lds_sync<WAVE>();
   }
 
@@ -526,7 +526,7 @@ col_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
       const double ky2 = kyv * kyv;
       const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp));
       if (kx == 0) {
-        const int kn = (N - ky) & (N - 1);
+        const int kn = neg_index<N>(ky);
         const CT syn = static_cast<CT>(dkval<N>(kn, inv_nsp));
         // rate on the packed line: i*kdy*Jy^ (k_dx = 0 for k_x = 0 and N/2)
         const cplx<CT> r = mul_i(sy * jy);

@@ -17,7 +17,8 @@
  *     Host buffers are copied; the caller keeps ownership.
  *   - Calls are synchronous on return.  One context per host thread; a
  *     context is not re-entrant.
- *   - Supported grids: N = 2^m, 16 <= N <= 4096, spacing > 0.
+ *   - Supported grids: N = 2^m with 16 <= N <= 4096, or N = 3*2^m with
+ *     48 <= N <= 768 (radix-3 first stage); spacing > 0.
  */
 #ifndef CSG_H
 #define CSG_H

@@ -1,7 +1,7 @@
 // Host-only check of the register Stockham engine (csg_fft.hpp): drives the
 // engine's stage_compute / plan / padding / Stockham output indexing over
 // emulated threads of one line and compares forward and inverse transforms
-// with a naive DFT for every line length the kernels use.  No GPU needed.
+// with a naive DFT for every line length the kernels use (2^m and 3*2^m).  No GPU needed.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -11,7 +11,7 @@
 
 using namespace csg;
 
-template <int DIR, int LEN, int S, class T>
+template <int DIR, int LEN, int S, class T, int NTAB>
 void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::vector<cplx<T>>& line) {
   constexpr bool INV = DIR > 0;
   constexpr int NS = fft_nstages(LEN);
@@ -22,7 +22,7 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
     for (int t = 0; t < TPL; ++t) {
       cplx<T> r[E];
       for (int j = 0; j < E; ++j) r[j] = v[t][j];
-      stage_compute<T, DIR, LEN, TPL, E, R, P, LEN>(r, t, tw);
+      stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB>(r, t, tw);
       for (int j = 0; j < E; ++j) v[t][j] = r[j];
     }
     if constexpr (S + 1 < NS) {
@@ -33,16 +33,18 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
       for (int t = 0; t < TPL; ++t)
         for (int j = 0; j < E; ++j) v[t][j] = line[pad<T>(t + TPL * j)];
     }
-    run_stages<DIR, LEN, S + 1, T>(v, tw, line);
+    run_stages<DIR, LEN, S + 1, T, NTAB>(v, tw, line);
   }
 }
 
-template <int LEN, class T>
+// NTAB: length of the twiddle table's full circle (LEN for column lines,
+// 2 LEN for the half-length complex transforms of the real row passes).
+template <int LEN, class T, int NTAB = LEN>
 int check(double tol) {
   constexpr int E = fft_E(LEN), TPL = LEN / E;
-  std::vector<cplx<T>> tw(LEN / 2);
-  for (int m = 0; m < LEN / 2; ++m) {
-    const long double a = -2.0L * 3.141592653589793238462643383279502884L * m / LEN;
+  std::vector<cplx<T>> tw(NTAB / 2);
+  for (int m = 0; m < NTAB / 2; ++m) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * m / NTAB;
     tw[m] = cplx<T>{static_cast<T>(cosl(a)), static_cast<T>(sinl(a))};
   }
   std::vector<std::complex<double>> x(LEN);
@@ -63,8 +65,8 @@ int check(double tol) {
         v[t][j] = cplx<T>{static_cast<T>(z.real()), static_cast<T>(z.imag())};
       }
     std::vector<cplx<T>> line(padded_len<T>(LEN));
-    if (dir < 0) run_stages<-1, LEN, 0, T>(v, tw.data(), line);
-    else run_stages<+1, LEN, 0, T>(v, tw.data(), line);
+    if (dir < 0) run_stages<-1, LEN, 0, T, NTAB>(v, tw.data(), line);
+    else run_stages<+1, LEN, 0, T, NTAB>(v, tw.data(), line);
     double err = 0, mag = 0;
     for (int k = 0; k < LEN; ++k) {
       std::complex<long double> acc = 0;
@@ -77,7 +79,7 @@ int check(double tol) {
       mag = std::fmax(mag, std::abs(std::complex<double>((double)acc.real(), (double)acc.imag())));
     }
     const bool ok = err <= tol * mag;
-    printf("LEN=%5d E=%2d stages=%d dir=%+d %s rel err %.3e\n", LEN, E, fft_nstages(LEN), dir,
+    printf("LEN=%5d NTAB=%5d E=%2d stages=%d dir=%+d %s rel err %.3e\n", LEN, NTAB, E, fft_nstages(LEN), dir,
            sizeof(T) == 8 ? "f64" : "f32", err / mag);
     if (!ok) { printf("  FAIL\n"); ++bad; }
   }
@@ -96,8 +98,25 @@ int main() {
   bad += check<1024, double>(1e-14);
   bad += check<2048, double>(1e-14);
   bad += check<4096, double>(1e-14);
+  bad += check<24, double>(1e-14);
+  bad += check<48, double>(1e-14);
+  bad += check<96, double>(1e-14);
+  bad += check<192, double>(1e-14);
+  bad += check<384, double>(1e-14);
+  bad += check<384, float>(2e-6);
+  bad += check<768, double>(1e-14);
   bad += check<1024, float>(2e-6);
   bad += check<4096, float>(2e-6);
+  // row lengths L = N/2 with the N-point table
+  bad += check<8, double, 16>(1e-14);
+  bad += check<512, double, 1024>(1e-14);
+  bad += check<2048, double, 4096>(1e-14);
+  bad += check<24, double, 48>(1e-14);
+  bad += check<48, double, 96>(1e-14);
+  bad += check<96, double, 192>(1e-14);
+  bad += check<192, double, 384>(1e-14);
+  bad += check<384, double, 768>(1e-14);
+  bad += check<192, float, 384>(2e-6);
   printf(bad ? "FAILED %d\n" : "ALL OK\n", bad);
   return bad ? 1 : 0;
 }

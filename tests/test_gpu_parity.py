@@ -100,6 +100,22 @@ def test_one_step_matches_oracle(pkg, n):
     assert np.max(np.abs(c1 - r1)) < 1e-14
 
 
+@pytest.mark.parametrize("n", [48, 96, 192, 384, 768])
+def test_radix3_grids_match_oracle(pkg, n):
+    """N = 3 * 2^m grids (radix-3 first stage): energy, one step, 100 steps."""
+    say(f"radix-3 grid n={n}")
+    c, phi = _fields(pkg, n)
+    shape, grid = pkg.geometry(phi), pkg.make_grid(n, 1.0)
+    ro, rg = o.geometry(phi, P), o.make_grid(n, 1.0)
+    assert abs(pkg.energy(c, shape, grid) / o.energy(c, ro, rg, P) - 1) < 1e-13
+    c1 = pkg.step(c, shape, grid, pkg.Parameters(), 0.01)
+    assert np.max(np.abs(c1 - o.step(c, ro, rg, P, 0.01))) < 1e-14
+    cg, hg = pkg.run_simulation(c, phi, steps=100)
+    cr, hr = o.run_simulation_fast(c, phi, steps=100)
+    assert np.max(np.abs(cg - cr)) < 1e-12
+    assert np.max(np.abs(hg - hr) / np.maximum(1.0, np.abs(hr))) < 1e-12
+
+
 def test_nondefault_parameters(pkg):
     say("non-default parameters")
     p = pkg.Parameters(A=1.5, B=6.0, chi=-0.3, P=2.0, ca=0.4, cb=0.1, cc=0.9, kappa=1.3, mobility=0.7, Q=0.9)

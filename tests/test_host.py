@@ -69,7 +69,7 @@ def test_initial_fields_known_answer_energy_via_oracle(pkg):
     assert abs(F0 / 329.48858453024934 - 1) < 2e-15
 
 
-@pytest.mark.parametrize("n", [8, 48, 100, 8192])
+@pytest.mark.parametrize("n", [8, 24, 100, 1536, 8192])
 def test_unsupported_grid_rejected_without_gpu(pkg, n):
     from csg_amd import _capi
     h = _capi._VP()
