@@ -13,6 +13,8 @@ Also reported:
   cpu_baseline  the FP64 numpy oracle (reference-equivalent: 17 complex
                 N^2 FFTs per iteration), 1 thread, on a bounded sample of the
                 same workload
+  ensemble      the uneven 256-particle ensemble (mixed and FP64 forms)
+  gradient      gradient evaluations/s of the 512^2 adjoint workload
 """
 from __future__ import annotations
 
@@ -49,6 +51,8 @@ def _args():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--ens-count", type=int, default=256, help="ensemble members (0: skip the ensemble)")
+    ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--grad-steps", type=int, default=2000,
                     help="steps of the 512^2 adjoint workload timed for gradient evaluations/s "
                          "(0 skips it)")
@@ -125,6 +129,50 @@ def gradient_rate(pkg, a):
                         f"{max(1, round(steps ** 0.5))}", "gradient_evals_per_s": 1.0 / secs,
             "seconds_per_gradient": secs, "loss": g.loss,
             "forward_equivalent_site_updates_per_s": 3 * n * n * steps / secs}
+
+
+def ensemble_rate(pkg, a):
+    """The uneven ensemble: a.ens_count members, N drawn from {128, 256, 384,
+    512}, R = 0.3 N, w = 2, P and chi within +-30% of the defaults, seed 5749 +
+    index, a.ens_steps steps at dt, history every step kept on the device.
+    Mixed form timed (FP64 beside it); the error of every member's mixed run
+    against its FP64 run is reported."""
+    ms = pkg.ensemble_members(a.ens_count)
+    fields = [pkg.member_fields(m) for m in ms]
+    sizes = sorted({m.n for m in ms})
+    sites = sum(m.n * m.n for m in ms)
+    res = {}
+    for form in ("mixed", "fp64"):
+        ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision=form)
+        for i, (c0, phi) in enumerate(fields):
+            ens.set_fields(i, c0, phi)
+        ens.run(a.dt, 64, histories=False)           # warm-up: graph capture, caches
+        for i, (c0, phi) in enumerate(fields):
+            ens.set_fields(i, c0, phi)
+        t0 = time.perf_counter()
+        hs = ens.run(a.dt, a.ens_steps)
+        secs = time.perf_counter() - t0
+        res[form] = dict(secs=secs, hist=hs, c=[ens.field(i) for i in range(len(ms))],
+                         bytes=ens.device_bytes())
+        ens.close()
+    iters = a.ens_steps + 1
+    out = {"workload": f"ensemble of {len(ms)} particles, N in {sizes} "
+                       f"({', '.join(f'{n}: {sum(m.n == n for m in ms)}' for n in sizes)}), R=0.3N, w=2, "
+                       f"P, chi within +-30%, {a.ens_steps} steps, dt={a.dt:g}, history every step",
+           "sites": sites}
+    for form, r in res.items():
+        byts = sites * sum(PASS_BYTES[form]) * iters   # packed line layout: no N/2+1 column
+        gbs = byts / r["secs"] / 1e9
+        out[form] = {"site_updates_per_s": sites * iters / r["secs"], "ms_per_step": 1e3 * r["secs"] / iters,
+                     "algorithmic_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "device_bytes": r["bytes"]}
+    dc = [float(np.max(np.abs(x - y))) for x, y in zip(res["mixed"]["c"], res["fp64"]["c"])]
+    de = [float(np.max(np.abs(x[:, 2] - y[:, 2]) / np.abs(y[:, 2])))
+          for x, y in zip(res["mixed"]["hist"], res["fp64"]["hist"])]
+    out["mixed"]["max_dc_vs_fp64"] = max(dc)
+    out["mixed"]["median_dc_vs_fp64"] = float(np.median(dc))
+    out["mixed"]["rel_energy_err_vs_fp64"] = max(de)
+    out["mixed"]["mass_drift"] = max(float(np.max(np.abs(h[:, 1] - h[0, 1]))) for h in res["mixed"]["hist"])
+    return out
 
 
 def short_horizon_error(pkg, form, c0, phi, a, steps=1000):
@@ -213,6 +261,8 @@ def main():
         "roofline": roof,
         "forms": out,
     }
+    if a.ens_count:
+        line["ensemble"] = ensemble_rate(pkg, a)
     if a.grad_steps:
         line["gradient"] = gradient_rate(pkg, a)
     if not a.no_cpu:

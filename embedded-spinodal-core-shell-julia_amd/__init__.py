@@ -4,7 +4,8 @@ The compute path is ``libcsg.so`` (HIP kernels behind the C ABI of
 ``include/csg.h``); this package binds it with ctypes and mirrors the API of
 ``core_shell.jl`` (``Parameters``, ``make_grid``, ``initial_fields``,
 ``geometry``, ``bulk_energy``, ``bulk_derivative``, ``chemical_potential``,
-``transport_rate``, ``step``, ``energy``, ``run_simulation``).
+``transport_rate``, ``step``, ``energy``, ``run_simulation``), plus the
+adjoint (``adjoint.gradient``) and ensembles (``run_ensemble``).
 """
 from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, Shape,  # noqa: F401
                          bulk_derivative, bulk_energy, chemical_potential, energy, geometry,
@@ -12,3 +13,5 @@ from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, S
 from ._capi import CsgError, lib  # noqa: F401
 from .writers import julia_float, writedlm  # noqa: F401
 from . import adjoint  # noqa: F401
+from .ensemble import (Ensemble, EnsembleNonFiniteError, Member, ensemble_members,  # noqa: F401
+                       member_fields, run_ensemble)

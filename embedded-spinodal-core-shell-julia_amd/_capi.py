@@ -29,6 +29,8 @@ EXPORTS = (
     "csg_transport_rate", "csg_energy", "csg_geometry", "csg_bulk_energy",
     "csg_bulk_derivative", "csg_initial_fields", "csg_julia_rand", "csg_time_passes",
     "csg_device_bytes", "csg_last_error", "csg_adjoint_forward", "csg_adjoint_backward",
+    "csg_batch_create", "csg_batch_destroy", "csg_batch_set_fields", "csg_batch_run",
+    "csg_batch_get_field", "csg_batch_nonfinite_step", "csg_batch_device_bytes",
 )
 
 
@@ -79,6 +81,14 @@ def lib():
         "csg_adjoint_forward": (ctypes.c_int, [_VP, dbl, i64, i64]),
         "csg_adjoint_backward": (ctypes.c_int, [_VP, _D, _D, _D]),
         "csg_last_error": (ctypes.c_char_p, []),
+        "csg_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), i32, ctypes.POINTER(i32), _D,
+                                            ctypes.POINTER(CsgParams), i32, i32]),
+        "csg_batch_destroy": (ctypes.c_int, [_VP]),
+        "csg_batch_set_fields": (ctypes.c_int, [_VP, i32, _D, _D]),
+        "csg_batch_run": (ctypes.c_int, [_VP, dbl, i64, i64, ctypes.POINTER(_D)]),
+        "csg_batch_get_field": (ctypes.c_int, [_VP, i32, _D]),
+        "csg_batch_nonfinite_step": (i64, [_VP, i32]),
+        "csg_batch_device_bytes": (i64, [_VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -29,6 +29,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 # translation units compile in parallel.
 UNITS = [
     ("csg_capi.hip", []),
+    ("csg_batch.hip", []),
     ("csg_ops_fp64_pow2.hip", []),
     ("csg_ops_fp64_pow3.hip", []),
     ("csg_ops_mixed_pow2.hip", []),

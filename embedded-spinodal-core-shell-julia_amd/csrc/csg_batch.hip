@@ -1,0 +1,453 @@
+// Ensembles: many independent instances of the time loop of run_simulation
+// (core_shell.jl:84-103) advanced together on one GPU.
+//
+// Instances are grouped by grid size into classes.  A class keeps each of its
+// device arrays as one slab (instance q at q times the per-instance size) and
+// runs every pass as ONE launch over all its instances (instance = grid y),
+// with per-instance parameters read from a device table; each launch then has
+// (tiles or lines) x instances workgroups.  The classes are independent: the
+// captured graph of 32 loop iterations forks one branch per class.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/csg.h"
+#include "csg_host.hpp"
+
+using namespace csg;
+
+#define HIPCHK CSG_HIPCHK
+
+namespace {
+
+struct BatchClass {
+  int n = 0;
+  int m = 0;                     // instances
+  Ops ops{};
+  std::vector<int> members;      // batch index of slab slot q
+  std::vector<DevParams> hparams;
+  DevParams* dparams = nullptr;  // device copy, m entries
+  hipStream_t stream = nullptr;  // branch stream
+  size_t nn = 0;                 // sites per instance
+  size_t tsize = 8;              // bytes per bulk real
+  void* S = nullptr;             // state slab (double)
+  void* A = nullptr;             // bulk slabs
+  void* Bb = nullptr;
+  void* phi = nullptr;
+  double* real = nullptr;        // one instance of host I/O staging
+  void* tw = nullptr;
+  void* tw64 = nullptr;
+  double* part = nullptr;        // m partial records
+  long long part_stride = 0;     // doubles per instance
+  long long* ctr = nullptr;      // 2 per instance
+  double* hist = nullptr;        // m x hist_cap x 6
+  size_t hist_cap = 0;           // rows per instance
+};
+
+}  // namespace
+
+struct csg_batch {
+  int count = 0;
+  int precision = 0;
+  int device = 0;
+  std::vector<BatchClass> cls;
+  std::vector<int> cls_of, slot_of;
+  std::vector<char> fields_set;
+  std::vector<long long> nonfinite;
+  hipStream_t stream = nullptr;  // origin of the fork/join
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t gexec = nullptr;
+  double gdt = 0.0;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int bmalloc(csg_batch* b, void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  b->bytes += bytes;
+  return CSG_OK;
+}
+
+// Pass arguments covering all instances of class k.
+PassArgs class_args(const BatchClass& k) {
+  PassArgs a{};
+  a.p = k.hparams[0];
+  a.phi = k.phi;
+  a.ninst = k.m;
+  a.is.params = k.dparams;
+  a.is.phi = static_cast<long long>(k.nn * k.tsize);
+  a.is.part = k.part_stride;
+  return a;
+}
+
+long long slab_stride(const BatchClass& k, const void* buf) {
+  return static_cast<long long>(k.nn * (buf == k.S ? sizeof(double) : k.tsize));
+}
+
+void bind_in(const BatchClass& k, PassArgs& a, const void* in0, const void* in1) {
+  a.in0 = in0;
+  a.in1 = in1;
+  a.is.in0 = in0 ? slab_stride(k, in0) : 0;
+  a.is.in1 = in1 ? slab_stride(k, in1) : 0;
+}
+void bind_out(const BatchClass& k, PassArgs& a, void* out0, void* out1) {
+  a.out0 = out0;
+  a.out1 = out1;
+  a.is.out0 = out0 ? slab_stride(k, out0) : 0;
+  a.is.out1 = out1 ? slab_stride(k, out1) : 0;
+}
+
+FinalArgs class_final(const BatchClass& k, double* hist, double dt) {
+  FinalArgs f{};
+  f.p1 = k.part;
+  f.p2 = k.part + 5 * k.ops.n_row1;
+  f.p3 = k.part + 5 * k.ops.n_row1 + k.ops.n_col;
+  f.n1 = k.ops.n_row1;
+  f.n2 = k.ops.n_col;
+  f.n3 = k.ops.n_row3;
+  f.hist = hist;
+  f.ctr = k.ctr;
+  f.dt = dt;
+  f.spacing2 = k.hparams[0].spacing * k.hparams[0].spacing;
+  f.inv_n2 = 1.0 / (static_cast<double>(k.n) * k.n);
+  f.kappa = k.hparams[0].kappa;
+  f.params = k.dparams;
+  f.s_part = k.part_stride;
+  f.s_hist = static_cast<long long>(k.hist_cap) * 6;
+  return f;
+}
+
+// One loop iteration (core_shell.jl:90-95) of every instance of class k.
+void enqueue_class_iteration(const BatchClass& k, double dt, bool advance, hipStream_t s) {
+  PassArgs a = class_args(k);
+  bind_in(k, a, k.S, nullptr);
+  bind_out(k, a, k.A, nullptr);
+  a.part = k.part;
+  k.ops.p1(a, k.tw, s);
+  a = class_args(k);
+  bind_in(k, a, k.S, k.A);
+  bind_out(k, a, k.A, k.Bb);
+  a.part = k.part + 5 * k.ops.n_row1;
+  k.ops.p2(a, k.tw, s);
+  a = class_args(k);
+  bind_in(k, a, k.A, k.Bb);
+  bind_out(k, a, k.A, k.Bb);
+  a.part = k.part + 5 * k.ops.n_row1 + k.ops.n_col;
+  k.ops.p3(a, k.tw, s);
+  if (advance) {
+    a = class_args(k);
+    bind_in(k, a, k.A, k.Bb);
+    bind_out(k, a, k.S, nullptr);
+    a.dt = dt;
+    a.fin = class_final(k, k.hist, dt);
+    k.ops.p4(a, k.tw, s);
+  } else {
+    hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(k.m), dim3(FIN_NT), 0, s, class_final(k, k.hist, dt));
+  }
+}
+
+// `iters` loop iterations of every class, one branch per class, forked from
+// and joined back into b->stream (captured or eager).
+int enqueue_iterations(csg_batch* b, double dt, int iters, bool last_advances) {
+  HIPCHK(hipEventRecord(b->fork, b->stream));
+  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
+    const BatchClass& k = b->cls[ci];
+    HIPCHK(hipStreamWaitEvent(k.stream, b->fork, 0));
+    for (int it = 0; it < iters; ++it) enqueue_class_iteration(k, dt, it + 1 < iters || last_advances, k.stream);
+    HIPCHK(hipEventRecord(b->join[ci], k.stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->join[ci], 0));
+  }
+  HIPCHK(hipGetLastError());
+  return CSG_OK;
+}
+
+int build_batch_graph(csg_batch* b, double dt) {
+  if (b->gexec && b->gdt == dt) return CSG_OK;
+  if (b->gexec) {
+    HIPCHK(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_iterations(b, dt, STEPS_PER_GRAPH, true);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->gexec = nullptr;
+    return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  b->gdt = dt;
+  return CSG_OK;
+}
+
+int ensure_batch_hist(csg_batch* b, size_t rows) {
+  for (BatchClass& k : b->cls) {
+    if (rows <= k.hist_cap) continue;
+    if (b->gexec) {  // the captured graph holds the old history pointers
+      HIPCHK(hipGraphExecDestroy(b->gexec));
+      b->gexec = nullptr;
+    }
+    if (k.hist) {
+      HIPCHK(hipFree(k.hist));
+      b->bytes -= k.hist_cap * 6 * sizeof(double) * k.m;
+    }
+    k.hist = nullptr;
+    k.hist_cap = 0;
+    if (int rc = bmalloc(b, reinterpret_cast<void**>(&k.hist), rows * 6 * sizeof(double) * k.m)) return rc;
+    k.hist_cap = rows;
+  }
+  return CSG_OK;
+}
+
+int check_batch(const csg_batch* b, int32_t i) {
+  if (!b) return fail(CSG_ERR_ARG, "null ensemble");
+  if (i < 0 || i >= b->count) return fail(CSG_ERR_ARG, "instance index out of range");
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const double* spacing,
+                     const csg_params* params, int32_t precision, int32_t device) {
+  if (!out) return fail(CSG_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (count < 1 || !n) return fail(CSG_ERR_ARG, "count must be >= 1 and n non-NULL");
+  if (precision != CSG_FP64 && precision != CSG_MIXED) return fail(CSG_ERR_ARG, "unknown precision");
+  for (int i = 0; i < count; ++i) {
+    const double sp = spacing ? spacing[i] : 1.0;
+    if (!(sp > 0.0) || !std::isfinite(sp)) return fail(CSG_ERR_ARG, "spacing must be > 0");
+    Ops probe{};
+    const bool ok = precision == CSG_FP64 ? ops_for<double>(n[i], &probe) : ops_for<float>(n[i], &probe);
+    if (!ok) return fail(CSG_ERR_UNSUPPORTED, unsupported_size_message(n[i]));
+  }
+  csg_batch* b = new csg_batch();
+  b->count = count;
+  b->precision = precision;
+  b->device = device;
+  b->cls_of.assign(count, -1);
+  b->slot_of.assign(count, -1);
+  b->fields_set.assign(count, 0);
+  b->nonfinite.assign(count, -1);
+  for (int i = 0; i < count; ++i) {
+    int ci = -1;
+    for (size_t k = 0; k < b->cls.size(); ++k)
+      if (b->cls[k].n == n[i]) ci = static_cast<int>(k);
+    if (ci < 0) {
+      b->cls.emplace_back();
+      ci = static_cast<int>(b->cls.size()) - 1;
+      b->cls[ci].n = n[i];
+    }
+    BatchClass& k = b->cls[ci];
+    b->cls_of[i] = ci;
+    b->slot_of[i] = k.m++;
+    k.members.push_back(i);
+    csg_params p;
+    if (params) p = params[i]; else csg_default_params(&p);
+    const double sp = spacing ? spacing[i] : 1.0;
+    k.hparams.push_back(DevParams{p.A, p.B, p.chi, p.P, p.ca, p.cb, p.cc, p.kappa, p.mobility, p.Q, sp,
+                                  (1.0 / sp) / n[i]});
+  }
+  auto cleanup = [&](int code) { csg_batch_destroy(b); return code; };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
+    return cleanup(fail(CSG_ERR_HIP, std::string("stream/event: ") + hipGetErrorString(e)));
+  b->join.assign(b->cls.size(), nullptr);
+  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
+    BatchClass& k = b->cls[ci];
+    const int nk = k.n;
+    if (precision == CSG_FP64) ops_for<double>(nk, &k.ops); else ops_for<float>(nk, &k.ops);
+    k.nn = static_cast<size_t>(nk) * nk;
+    k.tsize = precision == CSG_FP64 ? sizeof(double) : sizeof(float);
+    k.part_stride = 5LL * k.ops.n_row1 + k.ops.n_col + k.ops.n_row3;
+    if ((e = hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->join[ci], hipEventDisableTiming)) != hipSuccess)
+      return cleanup(fail(CSG_ERR_HIP, std::string("stream/event: ") + hipGetErrorString(e)));
+    int rc;
+    const size_t m = static_cast<size_t>(k.m);
+    if ((rc = bmalloc(b, &k.S, m * k.nn * sizeof(double)))) return cleanup(rc);
+    if ((rc = bmalloc(b, &k.A, m * k.nn * k.tsize))) return cleanup(rc);
+    if ((rc = bmalloc(b, &k.Bb, m * k.nn * k.tsize))) return cleanup(rc);
+    if ((rc = bmalloc(b, &k.phi, m * k.nn * k.tsize))) return cleanup(rc);
+    if ((rc = bmalloc(b, reinterpret_cast<void**>(&k.real), k.nn * sizeof(double)))) return cleanup(rc);
+    if ((rc = bmalloc(b, reinterpret_cast<void**>(&k.part), m * k.part_stride * sizeof(double)))) return cleanup(rc);
+    if ((rc = bmalloc(b, reinterpret_cast<void**>(&k.ctr), m * 2 * sizeof(long long)))) return cleanup(rc);
+    if ((rc = bmalloc(b, reinterpret_cast<void**>(&k.dparams), m * sizeof(DevParams)))) return cleanup(rc);
+    e = hipMemcpy(k.dparams, k.hparams.data(), m * sizeof(DevParams), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+    // twiddles exp(-2 pi i j / N), extended precision, rounded once
+    std::vector<double> tw64(2 * nk);
+    std::vector<float> tw32(2 * nk);
+    for (int j = 0; j < nk; ++j) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * j / nk;
+      tw64[2 * j] = static_cast<double>(cosl(ang));
+      tw64[2 * j + 1] = static_cast<double>(sinl(ang));
+      tw32[2 * j] = static_cast<float>(cosl(ang));
+      tw32[2 * j + 1] = static_cast<float>(sinl(ang));
+    }
+    if ((rc = bmalloc(b, &k.tw64, tw64.size() * sizeof(double)))) return cleanup(rc);
+    e = hipMemcpy(k.tw64, tw64.data(), tw64.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+    if (precision == CSG_FP64) {
+      k.tw = k.tw64;
+    } else {
+      if ((rc = bmalloc(b, &k.tw, tw32.size() * sizeof(float)))) return cleanup(rc);
+      e = hipMemcpy(k.tw, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+    }
+  }
+  *out = b;
+  return CSG_OK;
+}
+
+int csg_batch_destroy(csg_batch* b) {
+  if (!b) return CSG_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (BatchClass& k : b->cls)
+    if (k.stream) (void)hipStreamSynchronize(k.stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  for (BatchClass& k : b->cls) {
+    void* bufs[] = {k.S, k.A, k.Bb, k.phi, k.real, k.part, k.ctr, k.hist, k.dparams, k.tw64};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    if (k.tw && k.tw != k.tw64) (void)hipFree(k.tw);
+    if (k.stream) (void)hipStreamDestroy(k.stream);
+  }
+  for (hipEvent_t ev : b->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CSG_OK;
+}
+
+int64_t csg_batch_device_bytes(const csg_batch* b) { return b ? static_cast<int64_t>(b->bytes) : 0; }
+
+int csg_batch_set_fields(csg_batch* b, int32_t i, const double* c0, const double* phi) {
+  if (int rc = check_batch(b, i)) return rc;
+  if (!c0 || !phi) return fail(CSG_ERR_ARG, "c0 and phi must be non-NULL");
+  HIPCHK(hipSetDevice(b->device));
+  BatchClass& k = b->cls[b->cls_of[i]];
+  const size_t q = static_cast<size_t>(b->slot_of[i]);
+  char* phi_q = static_cast<char*>(k.phi) + q * k.nn * k.tsize;
+  if (b->precision == CSG_FP64) {
+    HIPCHK(hipMemcpyAsync(phi_q, phi, k.nn * sizeof(double), hipMemcpyHostToDevice, k.stream));
+  } else {
+    std::vector<float> tmp(k.nn);
+    for (size_t j = 0; j < k.nn; ++j) tmp[j] = static_cast<float>(phi[j]);
+    HIPCHK(hipMemcpyAsync(phi_q, tmp.data(), k.nn * sizeof(float), hipMemcpyHostToDevice, k.stream));
+    HIPCHK(hipStreamSynchronize(k.stream));
+  }
+  HIPCHK(hipMemcpyAsync(k.real, c0, k.nn * sizeof(double), hipMemcpyHostToDevice, k.stream));
+  PassArgs a{};
+  a.p = k.hparams[q];
+  a.in0 = k.real;
+  a.out0 = static_cast<char*>(k.S) + q * k.nn * sizeof(double);
+  k.ops.fwd(a, k.tw64, k.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(k.stream));
+  b->fields_set[i] = 1;
+  return CSG_OK;
+}
+
+int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, double* const* history) {
+  if (!b) return fail(CSG_ERR_ARG, "null ensemble");
+  for (int i = 0; i < b->count; ++i)
+    if (!b->fields_set[i]) return fail(CSG_ERR_STATE, "fields of instance " + std::to_string(i) + " not set");
+  if (steps < 0 || hist_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/hist_every/dt");
+  HIPCHK(hipSetDevice(b->device));
+  if (int rc = ensure_batch_hist(b, static_cast<size_t>(steps) + 1)) return rc;
+  for (BatchClass& k : b->cls) {
+    std::vector<long long> init(2 * static_cast<size_t>(k.m));
+    for (int q = 0; q < k.m; ++q) {
+      init[2 * q] = 0;
+      init[2 * q + 1] = -1;
+    }
+    HIPCHK(hipMemcpyAsync(k.ctr, init.data(), init.size() * sizeof(long long), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  int64_t done = 0;
+  if (steps >= 2 * STEPS_PER_GRAPH) {
+    if (int rc = build_batch_graph(b, dt)) return rc;
+    while (steps - done >= STEPS_PER_GRAPH) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+      done += STEPS_PER_GRAPH;
+    }
+  }
+  // the remaining iterations, then the last history row without a step
+  if (int rc = enqueue_iterations(b, dt, static_cast<int>(steps - done) + 1, false)) return rc;
+  const int64_t rows = steps / hist_every + 1;
+  std::vector<std::vector<long long>> ctr(b->cls.size());
+  std::vector<std::vector<double>> h(b->cls.size());
+  for (size_t ci = 0; ci < b->cls.size(); ++ci) {
+    const BatchClass& k = b->cls[ci];
+    ctr[ci].resize(2 * static_cast<size_t>(k.m));
+    HIPCHK(hipMemcpyAsync(ctr[ci].data(), k.ctr, ctr[ci].size() * sizeof(long long), hipMemcpyDeviceToHost,
+                          b->stream));
+    if (history) {
+      h[ci].resize(static_cast<size_t>(k.m) * k.hist_cap * 6);
+      HIPCHK(hipMemcpyAsync(h[ci].data(), k.hist, h[ci].size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  bool any_bad = false;
+  long long first_bad = -1;
+  int bad_inst = -1;
+  for (int i = 0; i < b->count; ++i) {
+    const size_t ci = static_cast<size_t>(b->cls_of[i]);
+    const size_t q = static_cast<size_t>(b->slot_of[i]);
+    b->nonfinite[i] = ctr[ci][2 * q + 1];
+    if (b->nonfinite[i] >= 0 && !any_bad) {
+      any_bad = true;
+      first_bad = b->nonfinite[i];
+      bad_inst = i;
+    }
+    if (history && history[i]) {
+      const double* src = h[ci].data() + q * b->cls[ci].hist_cap * 6;
+      for (int64_t r = 0; r < rows; ++r)
+        for (int col = 0; col < 6; ++col)
+          history[i][r + col * rows] = src[static_cast<size_t>(r * hist_every) * 6 + col];
+    }
+  }
+  if (any_bad)
+    return fail(CSG_ERR_NONFINITE, "Non-finite composition at step " + std::to_string(first_bad) + " (instance " +
+                                       std::to_string(bad_inst) + ")");
+  return CSG_OK;
+}
+
+int csg_batch_get_field(csg_batch* b, int32_t i, double* out) {
+  if (int rc = check_batch(b, i)) return rc;
+  if (!b->fields_set[i]) return fail(CSG_ERR_STATE, "fields not set");
+  if (!out) return fail(CSG_ERR_ARG, "out is NULL");
+  HIPCHK(hipSetDevice(b->device));
+  BatchClass& k = b->cls[b->cls_of[i]];
+  const size_t q = static_cast<size_t>(b->slot_of[i]);
+  PassArgs a{};
+  a.p = k.hparams[q];
+  a.in0 = static_cast<const char*>(k.S) + q * k.nn * sizeof(double);
+  a.out0 = k.real;
+  k.ops.inv_state(a, k.tw64, k.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, k.real, k.nn * sizeof(double), hipMemcpyDeviceToHost, k.stream));
+  HIPCHK(hipStreamSynchronize(k.stream));
+  return CSG_OK;
+}
+
+int64_t csg_batch_nonfinite_step(const csg_batch* b, int32_t i) {
+  if (!b || i < 0 || i >= b->count) return -1;
+  return b->nonfinite[i];
+}
+
+}  // extern "C"

@@ -11,36 +11,20 @@
 #include <vector>
 
 #include "../../include/csg.h"
-#include "csg_ops.hpp"
+#include "csg_host.hpp"
 
 using namespace csg;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int csg::fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIPCHK(expr)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(CSG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-  } while (0)
-
-template <class T>
-bool ops_for(int n, Ops* o) {
-  if constexpr (std::is_same<T, double>::value) return ops_fp64_pow2(n, o) || ops_fp64_pow3(n, o);
-  else return ops_mixed_pow2(n, o) || ops_mixed_pow3(n, o);
-}
-
-constexpr int FIN_NT = 256;
-constexpr int STEPS_PER_GRAPH = 32;
-
-}  // namespace
+#define HIPCHK CSG_HIPCHK
 
 struct csg_ctx {
   int n = 0;
@@ -264,8 +248,7 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
   const bool ok = (precision == CSG_FP64) ? ops_for<double>(n, &c->ops) : ops_for<float>(n, &c->ops);
   if (!ok) {
     delete c;
-    return fail(CSG_ERR_UNSUPPORTED, "grid size " + std::to_string(n) +
-                                         " not supported (N = 2^m with 16 <= N <= 4096, or N = 3*2^m with 48 <= N <= 768)");
+    return fail(CSG_ERR_UNSUPPORTED, unsupported_size_message(n));
   }
   c->dp = DevParams{c->p.A, c->p.B, c->p.chi, c->p.P, c->p.ca, c->p.cb, c->p.cc,
                     c->p.kappa, c->p.mobility, c->p.Q, spacing, (1.0 / spacing) / n};

@@ -62,6 +62,19 @@ struct FinalArgs {
   double spacing2;
   double inv_n2;      // 1 / N^2
   double kappa;
+  // ensembles: instance q uses p1..p3 + q*s_part, hist + q*s_hist,
+  // ctr + 2q and, when params is set, params[q].kappa / .spacing
+  const struct DevParams* params;
+  long long s_part, s_hist;
+};
+
+// Ensemble of same-size instances: instance q (blockIdx.y) finds its arrays
+// at the base pointers plus q times these byte strides, and its parameters at
+// params[q] (NULL: PassArgs::p for every instance).
+struct InstanceStrides {
+  const DevParams* params;
+  long long in0, in1, out0, out1, phi;   // bytes
+  long long part;                        // doubles
 };
 
 struct PassArgs {
@@ -74,7 +87,48 @@ struct PassArgs {
   double dt;
   DevParams p;
   FinalArgs fin;     // P4 only
+  InstanceStrides is;
+  int ninst;         // instances (gridDim.y); 0 or 1: a single field
 };
+
+__device__ __forceinline__ const void* offset_bytes(const void* p, long long q, long long s) {
+  return p ? static_cast<const char*>(p) + q * s : p;
+}
+__device__ __forceinline__ void* offset_bytes(void* p, long long q, long long s) {
+  return p ? static_cast<char*>(p) + q * s : p;
+}
+
+// FinalArgs of instance q.
+__device__ __forceinline__ FinalArgs final_view(const FinalArgs& f0, int q) {
+  FinalArgs f = f0;
+  if (!f0.hist) return f;   // no history row: nothing of the record is used
+  f.p1 = f0.p1 + q * f0.s_part;
+  f.p2 = f0.p2 + q * f0.s_part;
+  f.p3 = f0.p3 + q * f0.s_part;
+  f.hist = f0.hist + q * f0.s_hist;
+  f.ctr = f0.ctr + 2 * q;
+  if (f0.params) {
+    f.kappa = f0.params[q].kappa;
+    f.spacing2 = f0.params[q].spacing * f0.params[q].spacing;
+  }
+  return f;
+}
+
+// PassArgs of the instance this workgroup works on (blockIdx.y).
+__device__ __forceinline__ PassArgs instance_view(const PassArgs& a0) {
+  const int q = blockIdx.y;
+  if (q == 0 && !a0.is.params) return a0;
+  PassArgs a = a0;
+  a.in0 = offset_bytes(a0.in0, q, a0.is.in0);
+  a.in1 = offset_bytes(a0.in1, q, a0.is.in1);
+  a.out0 = offset_bytes(a0.out0, q, a0.is.out0);
+  a.out1 = offset_bytes(a0.out1, q, a0.is.out1);
+  a.phi = offset_bytes(a0.phi, q, a0.is.phi);
+  if (a0.part) a.part = a0.part + q * a0.is.part;
+  if (a0.is.params) a.p = a0.is.params[q];
+  a.fin = final_view(a0.fin, q);
+  return a;
+}
 
 template <class T> __device__ __forceinline__ bool finite(T v) { return isfinite(v); }
 
@@ -177,8 +231,9 @@ struct RowCfg {
 
 template <class CT, class TIN, class TOUT, int N, int MODE>
 __global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE>::NT))
-row_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
+row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
+  const PassArgs a = instance_view(a0);
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr int NARR = Cfg::NARR;
   constexpr bool WAVE = Cfg::WAVE;
@@ -401,8 +456,9 @@ struct ColCfg {
 
 template <class CT, class TS, int N, int MODE>
 __global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE>::NT))
-col_pass(PassArgs a, const cplx<CT>* __restrict__ tw_g) {
+col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   using Cfg = ColCfg<CT, TS, N, MODE>;
+  const PassArgs a = instance_view(a0);
   constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
   __shared__ cplx<CT> lds[C * LP];
@@ -602,11 +658,12 @@ pointwise_kernel(const double* __restrict__ c, const TP* __restrict__ phi, doubl
   }
 }
 
-// Standalone history reduction (the last row of a run, single evaluations).
+// Standalone history reduction (the last row of a run, single evaluations);
+// one workgroup per instance.
 template <int NT>
 __global__ void __launch_bounds__(NT) finalize_kernel(FinalArgs f) {
   __shared__ double scratch[8 * ((NT + 63) / 64)];
-  finalize_block<NT>(f, scratch);
+  finalize_block<NT>(final_view(f, blockIdx.x), scratch);
 }
 
 }  // namespace csg

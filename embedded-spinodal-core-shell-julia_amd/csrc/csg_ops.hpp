@@ -23,17 +23,20 @@ struct Ops {
   int n_adj_col = 0, n_adj_row = 0;
 };
 
+// Launches cover a.ninst instances (grid y) of the same size.
+inline unsigned instances(const PassArgs& a) { return a.ninst > 1 ? static_cast<unsigned>(a.ninst) : 1u; }
+
 template <class CT, class TIN, class TOUT, int N, int MODE>
 void launch_row(const PassArgs& a, const void* tw, hipStream_t s) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
-  hipLaunchKernelGGL((row_pass<CT, TIN, TOUT, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
+  hipLaunchKernelGGL((row_pass<CT, TIN, TOUT, N, MODE>), dim3(Cfg::GRID, instances(a)), dim3(Cfg::NT), 0, s, a,
                      static_cast<const cplx<CT>*>(tw));
 }
 
 template <class CT, class TS, int N, int MODE>
 void launch_col(const PassArgs& a, const void* tw, hipStream_t s) {
   using Cfg = ColCfg<CT, TS, N, MODE>;
-  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE>), dim3(Cfg::GRID), dim3(Cfg::NT), 0, s, a,
+  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE>), dim3(Cfg::GRID, instances(a)), dim3(Cfg::NT), 0, s, a,
                      static_cast<const cplx<CT>*>(tw));
 }
 

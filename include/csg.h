@@ -144,6 +144,40 @@ int csg_adjoint_backward(csg_ctx* ctx, const double* dL_dcT, double* dL_dc0, dou
 /* Bytes of device memory held by the context. */
 int64_t csg_device_bytes(const csg_ctx* ctx);
 
+/* ---- ensembles ------------------------------------------------------------
+ * No counterpart in the reference, which runs one particle per call: an
+ * ensemble is `count` independent instances of run_simulation
+ * (core_shell.jl:84-103), each with its own grid size, spacing, parameters,
+ * c0 and phi, advanced together.  Instances of equal size share batched
+ * kernel launches (one instance per grid row of workgroups); the sizes run as
+ * parallel branches of one captured graph per 32 loop iterations.  Same
+ * supported sizes and precisions as csg_create. */
+typedef struct csg_batch csg_batch;
+
+/* n[i], spacing[i] (NULL: all 1.0), params[i] (NULL: all defaults). */
+int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const double* spacing,
+                     const csg_params* params, int32_t precision, int32_t device);
+int csg_batch_destroy(csg_batch* b);
+
+/* Upload instance i's c0 and phi (host, column-major n[i] x n[i]; copied). */
+int csg_batch_set_fields(csg_batch* b, int32_t i, const double* c0, const double* phi);
+
+/* csg_run for every instance: history[i] (may be NULL, as may `history`)
+ * receives instance i's (rows x 6) column-major history as in csg_run.  An
+ * instance whose composition turns non-finite keeps its own record
+ * (csg_batch_nonfinite_step) and does not stop the others; the call then
+ * returns CSG_ERR_NONFINITE after all instances have run. */
+int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, double* const* history);
+
+/* Download instance i's composition field (column-major n[i] x n[i]). */
+int csg_batch_get_field(csg_batch* b, int32_t i, double* c_out);
+
+/* First non-finite step of instance i in the last csg_batch_run, or -1. */
+int64_t csg_batch_nonfinite_step(const csg_batch* b, int32_t i);
+
+/* Bytes of device memory held by the ensemble. */
+int64_t csg_batch_device_bytes(const csg_batch* b);
+
 const char* csg_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,150 @@
+"""Ensembles (csg_batch_*): every member equals the single-field path on the
+same inputs bit for bit (same kernels, one instance per workgroup row), and the
+FP64 ensemble matches the oracle's run_simulation; argument checks on the CPU.
+"""
+import ctypes
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import core_shell_oracle as o
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _members(pkg):
+    d = pkg.Parameters()
+    spec = [(64, 5749, 1.0, 1.0), (48, 11, 1.2, 0.8), (96, 12, 0.7, 1.3), (64, 13, 1.3, 0.7),
+            (128, 14, 0.9, 1.1), (16, 15, 1.0, 1.0), (48, 16, 1.1, 0.9)]
+    out = []
+    for n, seed, fp, fchi in spec:
+        out.append(pkg.Member(n=n, radius=0.3 * n, width=2.0, seed=seed,
+                              p=dataclasses.replace(d, P=d.P * fp, chi=d.chi * fchi)))
+    return out
+
+
+# ---- CPU -------------------------------------------------------------------
+
+def test_batch_arguments_rejected_without_gpu(pkg):
+    from csg_amd import _capi
+    L = pkg.lib()
+    h = _capi._VP()
+    ns = (ctypes.c_int32 * 3)(64, 100, 128)
+    assert L.csg_batch_create(h, 3, ns, None, None, 0, 0) == _capi.CSG_ERR_UNSUPPORTED
+    assert "not supported" in L.csg_last_error().decode()
+    ok = (ctypes.c_int32 * 2)(64, 128)
+    assert L.csg_batch_create(h, 0, ok, None, None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_batch_create(h, 2, None, None, None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_batch_create(None, 2, ok, None, None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_batch_create(h, 2, ok, None, None, 5, 0) == _capi.CSG_ERR_ARG
+    sp = np.array([1.0, -1.0])
+    assert L.csg_batch_create(h, 2, ok, _capi.ptr(sp), None, 0, 0) == _capi.CSG_ERR_ARG
+    assert L.csg_batch_run(None, 0.01, 10, 1, None) == _capi.CSG_ERR_ARG
+    assert L.csg_batch_nonfinite_step(None, 0) == -1
+    assert L.csg_batch_destroy(None) == 0
+
+
+def test_ensemble_members_workload(pkg):
+    ms = pkg.ensemble_members(256)
+    assert len(ms) == 256 and ms == pkg.ensemble_members(256)   # deterministic
+    d = pkg.Parameters()
+    sizes = [m.n for m in ms]
+    assert set(sizes) == {128, 256, 384, 512}
+    assert min(sizes.count(s) for s in (128, 256, 384, 512)) > 40
+    for i, m in enumerate(ms):
+        assert m.radius == 0.3 * m.n and m.width == 2.0 and m.seed == 5749 + i
+        assert 0.7 * d.P <= m.p.P <= 1.3 * d.P and 0.7 * d.chi <= m.p.chi <= 1.3 * d.chi
+        assert dataclasses.replace(m.p, P=d.P, chi=d.chi) == d
+
+
+# ---- GPU -------------------------------------------------------------------
+
+def _single(pkg, m, fields, steps, precision, dt=0.01):
+    c0, phi = fields
+    s = pkg.Session(m.n, m.spacing, m.p, precision)
+    s.set_fields(c0, phi)
+    h = s.run(dt, steps)
+    c = s.field()
+    s.close()
+    return c, h
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("steps", [100, 7])
+def test_ensemble_members_equal_single_runs(pkg, precision, steps):
+    """Uneven sizes (16..128, radix 2 and 3), per-member P/chi and seeds;
+    100 steps use the captured graph plus an eager tail, 7 steps only eager."""
+    ms = _members(pkg)
+    say(f"ensemble vs single, {precision}, {steps} steps, sizes {[m.n for m in ms]}")
+    fields = [pkg.member_fields(m) for m in ms]
+    ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision=precision)
+    for i, (c0, phi) in enumerate(fields):
+        ens.set_fields(i, c0, phi)
+    hs = ens.run(0.01, steps)
+    for i, m in enumerate(ms):
+        c1, h1 = _single(pkg, m, fields[i], steps, precision)
+        assert np.array_equal(ens.field(i), c1), (i, m.n)
+        assert np.array_equal(hs[i], h1), (i, m.n)
+    # a second run continues from the resident fields, like csg_run
+    hs2 = ens.run(0.01, 3)
+    assert np.array_equal(hs2[0][0, 1:], hs[0][-1, 1:])
+    ens.close()
+
+
+@pytest.mark.gpu
+def test_ensemble_fp64_matches_oracle(pkg):
+    ms = _members(pkg)[:4]
+    say("ensemble fp64 vs oracle, 200 steps")
+    fields = [pkg.member_fields(m) for m in ms]
+    res = pkg.run_ensemble([f[0] for f in fields], [f[1] for f in fields], steps=200,
+                           params=[m.p for m in ms], precision="fp64")
+    for (c, h), m, (c0, phi) in zip(res, ms, fields):
+        po = o.Parameters(**dataclasses.asdict(m.p))
+        cr, hr = o.run_simulation_fast(np.asarray(c0), np.asarray(phi), steps=200, p=po)
+        assert np.max(np.abs(c - cr)) < 1e-12, m.n
+        assert np.max(np.abs(h - hr) / np.maximum(1.0, np.abs(hr))) < 1e-12, m.n
+
+
+@pytest.mark.gpu
+def test_ensemble_history_every_and_spacing(pkg):
+    say("ensemble history_every=5, spacing 0.5/2.0")
+    d = pkg.Parameters()
+    ns, sps = [32, 64], [0.5, 2.0]
+    fields = [pkg.initial_fields(n, spacing=sp, radius=0.3 * n * sp, width=2.0 * sp) for n, sp in zip(ns, sps)]
+    ens = pkg.Ensemble(ns, spacings=sps, params=[d, d], precision="fp64")
+    for i, (c0, phi) in enumerate(fields):
+        ens.set_fields(i, c0, phi)
+    hs = ens.run(0.01, 70, history_every=5)
+    for i, (n, sp) in enumerate(zip(ns, sps)):
+        s = pkg.Session(n, sp, d, "fp64")
+        s.set_fields(*fields[i])
+        h = s.run(0.01, 70, history_every=5)
+        assert hs[i].shape == (15, 6) and np.array_equal(hs[i], h)
+        assert np.array_equal(ens.field(i), s.field())
+        s.close()
+    ens.close()
+
+
+@pytest.mark.gpu
+def test_ensemble_nonfinite_member_does_not_stop_others(pkg):
+    say("ensemble with a non-finite member")
+    ms = _members(pkg)[:3]
+    fields = [pkg.member_fields(m) for m in ms]
+    bad = np.array(fields[1][0], order="F", copy=True)
+    bad[3, 5] = np.nan
+    ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision="fp64")
+    for i, (c0, phi) in enumerate(fields):
+        ens.set_fields(i, bad if i == 1 else c0, phi)
+    with pytest.raises(pkg.EnsembleNonFiniteError) as ei:
+        ens.run(0.01, 40)
+    assert ei.value.steps == [-1, 0, -1] and ei.value.members == [1]
+    for i in (0, 2):
+        c1, _ = _single(pkg, ms[i], fields[i], 40, "fp64")
+        assert np.array_equal(ens.field(i), c1)
+    ens.close()
