@@ -12,6 +12,6 @@ from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, S
                          initial_fields, make_grid, run_simulation, step, transport_rate)
 from ._capi import CsgError, lib  # noqa: F401
 from .writers import julia_float, writedlm  # noqa: F401
-from . import adjoint  # noqa: F401
+from . import adjoint, schedules  # noqa: F401
 from .ensemble import (Ensemble, EnsembleNonFiniteError, Member, ensemble_members,  # noqa: F401
                        member_fields, run_ensemble)

@@ -31,6 +31,8 @@ EXPORTS = (
     "csg_device_bytes", "csg_last_error", "csg_adjoint_forward", "csg_adjoint_backward",
     "csg_batch_create", "csg_batch_destroy", "csg_batch_set_fields", "csg_batch_run",
     "csg_batch_get_field", "csg_batch_nonfinite_step", "csg_batch_device_bytes",
+    "csg_schedule_info", "csg_get_schedule", "csg_set_schedule", "csg_autotune",
+    "csg_batch_get_schedule", "csg_batch_set_schedule", "csg_batch_autotune",
 )
 
 
@@ -89,6 +91,13 @@ def lib():
         "csg_batch_get_field": (ctypes.c_int, [_VP, i32, _D]),
         "csg_batch_nonfinite_step": (i64, [_VP, i32]),
         "csg_batch_device_bytes": (i64, [_VP]),
+        "csg_schedule_info": (ctypes.c_int, [_VP, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "csg_get_schedule": (ctypes.c_int, [_VP, ctypes.POINTER(i32)]),
+        "csg_set_schedule": (ctypes.c_int, [_VP, ctypes.POINTER(i32)]),
+        "csg_autotune": (ctypes.c_int, [_VP, dbl, i32, ctypes.POINTER(i32), _D]),
+        "csg_batch_get_schedule": (ctypes.c_int, [_VP, i32, ctypes.POINTER(i32)]),
+        "csg_batch_set_schedule": (ctypes.c_int, [_VP, i32, ctypes.POINTER(i32)]),
+        "csg_batch_autotune": (ctypes.c_int, [_VP, dbl, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -30,11 +30,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 UNITS = [
     ("csg_capi.hip", []),
     ("csg_batch.hip", []),
-    ("csg_ops_fp64_pow2.hip", []),
-    ("csg_ops_fp64_pow3.hip", []),
-    ("csg_ops_mixed_pow2.hip", []),
-    ("csg_ops_mixed_pow3.hip", []),
-]
+] + [(f"csg_ops_{prec}_{group}.hip", []) for prec in ("fp64", "mixed") for group in ("small", "mid", "big", "pow3")]
 
 
 def _run(cmd):
@@ -55,7 +51,7 @@ def build(verbose: bool = False) -> str:
     rng_obj = os.path.join(BUILD, "julia_rng.o")
     jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "julia_rng.cpp"), "-o", rng_obj])
     objs.append(rng_obj)
-    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+    with ThreadPoolExecutor(max_workers=min(int(os.environ.get("CSG_BUILD_JOBS", "8")), len(jobs))) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:
         for o in outs:

@@ -12,6 +12,7 @@ reproduce the reference: ``precision`` ("fp64" or "mixed"), ``history_every``
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import os
 import weakref
@@ -97,11 +98,24 @@ def initial_fields(n: int, spacing: float = 1.0, radius: float = 70.0, width: fl
 _PREC = {"fp64": _capi.CSG_FP64, "mixed": _capi.CSG_MIXED}
 
 
+PASSES = ("P1_row", "P2_col", "P3_row", "P4_col")
+
+
+def _sel4(sel) -> "ctypes.Array":
+    if len(sel) != 4:
+        raise ValueError("a schedule selects one candidate for each of the 4 passes")
+    return (ctypes.c_int32 * 4)(*(int(x) for x in sel))
+
+
 class Session:
-    """One device context (grid size, spacing, parameters, precision)."""
+    """One device context (grid size, spacing, parameters, precision).
+
+    ``schedule``: None (the default candidates), "cached" (the tuned choice
+    stored by ``schedules.save`` for this size and precision, if any) or four
+    candidate indices (see ``schedule_info``)."""
 
     def __init__(self, n: int, spacing: float = 1.0, p: Parameters = Parameters(),
-                 precision: str = "fp64", device: int = 0):
+                 precision: str = "fp64", device: int = 0, schedule=None):
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {sorted(_PREC)}")
         self.n, self.spacing, self.p, self.precision = int(n), float(spacing), p, precision
@@ -111,6 +125,40 @@ class Session:
         self.handle = h
         self._phi_key = None
         self._fin = weakref.finalize(self, lib().csg_destroy, h)
+        if schedule == "cached":
+            from . import schedules
+            schedule = schedules.load(self.n, precision)
+        if schedule is not None:
+            self.set_schedule(schedule)
+
+    def schedule_info(self) -> list:
+        """Per pass, the compiled candidates as dicts (E = elements per thread,
+        W = rows or lines per workgroup, G = tile-load group, PF = prefetch)."""
+        out = []
+        for q in range(4):
+            cnt = ctypes.c_int32()
+            check(lib().csg_schedule_info(self.handle, q, ctypes.byref(cnt), None))
+            buf = (ctypes.c_int32 * (4 * cnt.value))()
+            check(lib().csg_schedule_info(self.handle, q, ctypes.byref(cnt), buf))
+            out.append([dict(zip(("E", "W", "G", "PF"), buf[4 * i:4 * i + 4])) for i in range(cnt.value)])
+        return out
+
+    @property
+    def schedule(self) -> tuple:
+        sel = (ctypes.c_int32 * 4)()
+        check(lib().csg_get_schedule(self.handle, sel))
+        return tuple(sel)
+
+    def set_schedule(self, sel):
+        check(lib().csg_set_schedule(self.handle, _sel4(sel)))
+
+    def autotune(self, dt: float = 0.01, rounds: int = 2):
+        """Time the candidates inside the captured loop (needs set_fields; the
+        resident field is kept): returns (selection, device us per step)."""
+        sel = (ctypes.c_int32 * 4)()
+        us = ctypes.c_double()
+        check(lib().csg_autotune(self.handle, float(dt), int(rounds), sel, ctypes.byref(us)))
+        return tuple(sel), us.value
 
     def close(self):
         self._fin()
@@ -259,13 +307,18 @@ def step(c, shape: Shape, grid: Grid, p: Parameters, dt: float, *, precision="fp
 
 def run_simulation(c0, phi, dt: float = 0.01, steps: int = 1000, spacing: float = 1.0,
                    p: Parameters = Parameters(), directory=None, *, precision: str = "fp64",
-                   history_every: int = 1, device: int = 0):
+                   history_every: int = 1, device: int = 0, schedule=None):
     """``run_simulation`` (core_shell.jl:84-103): returns (c, history) with
     history rows [t, mean c, F, min c, max c, dissipation] of c_s for
     s = 0, history_every, ..., before c_s is advanced.  c0 is not modified.
-    Raises NonFiniteCompositionError (and writes no files) on non-finite c_s."""
+    Raises NonFiniteCompositionError (and writes no files) on non-finite c_s.
+    ``schedule``: as for Session (None: the default kernel schedules)."""
     c0 = _fortran(c0)
     s = _session(c0.shape[0], spacing, p, precision, device)
+    if schedule == "cached":
+        from . import schedules
+        schedule = schedules.load(s.n, precision)
+    s.set_schedule((0, 0, 0, 0) if schedule is None else schedule)
     s.set_fields(c0, phi)
     hist = s.run(dt, steps, history_every)
     c = s.field()

@@ -270,7 +270,7 @@ int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const dou
     if (precision == CSG_FP64) ops_for<double>(nk, &k.ops); else ops_for<float>(nk, &k.ops);
     k.nn = static_cast<size_t>(nk) * nk;
     k.tsize = precision == CSG_FP64 ? sizeof(double) : sizeof(float);
-    k.part_stride = 5LL * k.ops.n_row1 + k.ops.n_col + k.ops.n_row3;
+    k.part_stride = 5LL * k.ops.max_row1 + k.ops.max_col + k.ops.max_row3;
     if ((e = hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->join[ci], hipEventDisableTiming)) != hipSuccess)
       return cleanup(fail(CSG_ERR_HIP, std::string("stream/event: ") + hipGetErrorString(e)));
@@ -448,6 +448,134 @@ int csg_batch_get_field(csg_batch* b, int32_t i, double* out) {
 int64_t csg_batch_nonfinite_step(const csg_batch* b, int32_t i) {
   if (!b || i < 0 || i >= b->count) return -1;
   return b->nonfinite[i];
+}
+
+}  // extern "C"
+
+// ---- kernel schedules per size class ------------------------------------------
+
+namespace {
+
+BatchClass* find_class(csg_batch* b, int n) {
+  for (BatchClass& k : b->cls)
+    if (k.n == n) return &k;
+  return nullptr;
+}
+
+constexpr int BATCH_TUNE_REPS = 3;
+
+// Mean device time per loop iteration of class k alone (its own captured
+// 32-iteration loop) with schedules sel.
+int time_class(BatchClass& k, double dt, const int sel[4], hipEvent_t e0, hipEvent_t e1, double* us) {
+  select_schedule(k.ops, sel);
+  std::vector<long long> init(2 * static_cast<size_t>(k.m));
+  for (int q = 0; q < k.m; ++q) {
+    init[2 * q] = 0;
+    init[2 * q + 1] = -1;
+  }
+  hipGraph_t g;
+  hipGraphExec_t ge = nullptr;
+  HIPCHK(hipStreamBeginCapture(k.stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < STEPS_PER_GRAPH; ++it) enqueue_class_iteration(k, dt, true, k.stream);
+  HIPCHK(hipStreamEndCapture(k.stream, &g));
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  auto reset = [&]() {
+    return hipMemcpyAsync(k.ctr, init.data(), init.size() * sizeof(long long), hipMemcpyHostToDevice, k.stream);
+  };
+  float ms = 0.f;
+  if ((e = reset()) == hipSuccess && (e = hipGraphLaunch(ge, k.stream)) == hipSuccess && (e = reset()) == hipSuccess &&
+      (e = hipEventRecord(e0, k.stream)) == hipSuccess) {
+    for (int r = 0; r < BATCH_TUNE_REPS && e == hipSuccess; ++r) e = hipGraphLaunch(ge, k.stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, k.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  }
+  (void)hipGraphExecDestroy(ge);
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("tuning run: ") + hipGetErrorString(e));
+  *us = 1e3 * ms / (BATCH_TUNE_REPS * STEPS_PER_GRAPH);
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_batch_get_schedule(const csg_batch* b, int32_t n, int32_t* sel) {
+  if (!b || !sel) return fail(CSG_ERR_ARG, "null ensemble or sel");
+  const BatchClass* k = find_class(const_cast<csg_batch*>(b), n);
+  if (!k) return fail(CSG_ERR_ARG, "no member of size " + std::to_string(n));
+  for (int q = 0; q < 4; ++q) sel[q] = k->ops.sel[q];
+  return CSG_OK;
+}
+
+int csg_batch_set_schedule(csg_batch* b, int32_t n, const int32_t* sel) {
+  if (!b || !sel) return fail(CSG_ERR_ARG, "null ensemble or sel");
+  BatchClass* k = find_class(b, n);
+  if (!k) return fail(CSG_ERR_ARG, "no member of size " + std::to_string(n));
+  const int s[4] = {sel[0], sel[1], sel[2], sel[3]};
+  if (!select_schedule(k->ops, s)) return fail(CSG_ERR_ARG, "schedule index out of range");
+  if (b->gexec) {
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  return CSG_OK;
+}
+
+int csg_batch_autotune(csg_batch* b, double dt, int32_t rounds) {
+  if (!b) return fail(CSG_ERR_ARG, "null ensemble");
+  for (int i = 0; i < b->count; ++i)
+    if (!b->fields_set[i]) return fail(CSG_ERR_STATE, "fields of instance " + std::to_string(i) + " not set");
+  if (rounds < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "rounds >= 1 and finite dt required");
+  HIPCHK(hipSetDevice(b->device));
+  if (b->gexec) {
+    HIPCHK(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  if (int rc = ensure_batch_hist(b, static_cast<size_t>(STEPS_PER_GRAPH) * BATCH_TUNE_REPS + 1)) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = CSG_OK;
+  for (BatchClass& k : b->cls) {
+    // the resident fields of the class are kept and restored afterwards
+    const size_t bytes = static_cast<size_t>(k.m) * k.nn * sizeof(double);
+    void* saved = nullptr;
+    hipError_t e = hipMalloc(&saved, bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(saved, k.S, bytes, hipMemcpyDeviceToDevice, k.stream);
+    if (e != hipSuccess) {
+      rc = fail(CSG_ERR_HIP, std::string("tuning buffer: ") + hipGetErrorString(e));
+      if (saved) (void)hipFree(saved);
+      break;
+    }
+    int best[4] = {k.ops.sel[0], k.ops.sel[1], k.ops.sel[2], k.ops.sel[3]};
+    double best_us = 0.0;
+    rc = time_class(k, dt, best, e0, e1, &best_us);
+    for (int r = 0; r < rounds && rc == CSG_OK; ++r)
+      for (int q = 0; q < 4 && rc == CSG_OK; ++q)
+        for (int i = 0; i < k.ops.cand[q].n && rc == CSG_OK; ++i) {
+          if (i == best[q]) continue;
+          int trial[4] = {best[0], best[1], best[2], best[3]};
+          trial[q] = i;
+          double us = 0.0;
+          rc = time_class(k, dt, trial, e0, e1, &us);
+          if (rc == CSG_OK && us < best_us) {
+            best_us = us;
+            best[q] = i;
+          }
+        }
+    select_schedule(k.ops, best);
+    e = hipMemcpyAsync(k.S, saved, bytes, hipMemcpyDeviceToDevice, k.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(k.stream);
+    (void)hipFree(saved);
+    if (rc == CSG_OK && e != hipSuccess) rc = fail(CSG_ERR_HIP, std::string("restore: ") + hipGetErrorString(e));
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // extern "C"

@@ -265,7 +265,7 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
   if ((rc = dmalloc(c, &c->Bb, nn * c->tsize))) return cleanup(rc);
   if ((rc = dmalloc(c, &c->phi, nn * c->tsize))) return cleanup(rc);
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->real), nn * sizeof(double)))) return cleanup(rc);
-  const size_t npart = 5 * c->ops.n_row1 + c->ops.n_col + c->ops.n_row3;
+  const size_t npart = 5 * c->ops.max_row1 + c->ops.max_col + c->ops.max_row3;
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->part), npart * sizeof(double)))) return cleanup(rc);
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->ctr), 2 * sizeof(long long)))) return cleanup(rc);
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->scratch_row), 8 * sizeof(double)))) return cleanup(rc);
@@ -750,6 +750,122 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
     dL_dparams[1] = -c->p.chi * srg;
     dL_dparams[2] = -c->p.P * srg;
   }
+  return CSG_OK;
+}
+
+}  // extern "C"
+
+// ---- kernel schedules and the runtime tuner ----------------------------------
+
+namespace {
+
+constexpr int TUNE_REPS = 4;   // timed graph replays (32 loop iterations each) per candidate
+
+// Mean device time per loop iteration of the captured loop with schedules
+// sel[0..3]; the history counter is reset so the rows stay in the buffer.
+int time_schedule(csg_ctx* c, double dt, const int sel[4], hipEvent_t e0, hipEvent_t e1, double* us) {
+  select_schedule(c->ops, sel);
+  if (c->gexec) {
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  if (int rc = build_graph(c, dt)) return rc;
+  if (int rc = reset_ctr(c)) return rc;
+  HIPCHK(hipGraphLaunch(c->gexec, c->stream));   // warm-up
+  if (int rc = reset_ctr(c)) return rc;
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < TUNE_REPS; ++r) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *us = 1e3 * ms / (TUNE_REPS * STEPS_PER_GRAPH);
+  return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_schedule_info(const csg_ctx* c, int32_t pass, int32_t* count, int32_t* sched) {
+  if (!c) return fail(CSG_ERR_ARG, "null context");
+  if (pass < 0 || pass > 3 || !count) return fail(CSG_ERR_ARG, "pass must be 0..3 and count non-NULL");
+  const PassCands& pc = c->ops.cand[pass];
+  *count = pc.n;
+  if (sched)
+    for (int i = 0; i < pc.n; ++i)
+      for (int q = 0; q < 4; ++q) sched[4 * i + q] = pc.sched[i][q];
+  return CSG_OK;
+}
+
+int csg_get_schedule(const csg_ctx* c, int32_t* sel) {
+  if (!c || !sel) return fail(CSG_ERR_ARG, "null context or sel");
+  for (int q = 0; q < 4; ++q) sel[q] = c->ops.sel[q];
+  return CSG_OK;
+}
+
+int csg_set_schedule(csg_ctx* c, const int32_t* sel) {
+  if (!c || !sel) return fail(CSG_ERR_ARG, "null context or sel");
+  const int s[4] = {sel[0], sel[1], sel[2], sel[3]};
+  bool same = true;
+  for (int q = 0; q < 4; ++q) same = same && s[q] == c->ops.sel[q];
+  if (same) return CSG_OK;
+  if (!select_schedule(c->ops, s)) return fail(CSG_ERR_ARG, "schedule index out of range (csg_schedule_info)");
+  if (c->gexec) {   // the captured loop holds the previous kernels
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  return CSG_OK;
+}
+
+int csg_autotune(csg_ctx* c, double dt, int32_t rounds, int32_t* sel_out, double* us_per_step) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
+  if (rounds < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "rounds >= 1 and finite dt required");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nn = static_cast<size_t>(c->n) * c->n;
+  if (!c->S2) {
+    if (int rc = dmalloc(c, &c->S2, nn * sizeof(double))) return rc;
+  }
+  // the resident field is kept and restored afterwards
+  HIPCHK(hipMemcpyAsync(c->S2, c->S, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (int rc = ensure_hist(c, static_cast<size_t>(STEPS_PER_GRAPH) * TUNE_REPS + 1)) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best[4] = {c->ops.sel[0], c->ops.sel[1], c->ops.sel[2], c->ops.sel[3]};
+  double best_us = 0.0;
+  int rc = time_schedule(c, dt, best, e0, e1, &best_us);
+  // coordinate descent over the four passes
+  for (int r = 0; r < rounds && rc == CSG_OK; ++r) {
+    for (int q = 0; q < 4 && rc == CSG_OK; ++q) {
+      for (int i = 0; i < c->ops.cand[q].n && rc == CSG_OK; ++i) {
+        if (i == best[q]) continue;
+        int trial[4] = {best[0], best[1], best[2], best[3]};
+        trial[q] = i;
+        double us = 0.0;
+        rc = time_schedule(c, dt, trial, e0, e1, &us);
+        if (rc == CSG_OK && us < best_us) {
+          best_us = us;
+          best[q] = i;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  select_schedule(c->ops, best);
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  HIPCHK(hipMemcpyAsync(c->S, c->S2, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  if (sel_out)
+    for (int q = 0; q < 4; ++q) sel_out[q] = best[q];
+  if (us_per_step) *us_per_step = best_us;
   return CSG_OK;
 }
 

@@ -150,21 +150,21 @@ template <int R, int DIR, int ST, class T> CSG_HD void dft(cplx<T>* x) {
 }
 
 // ---- plans -----------------------------------------------------------------
-// Elements per thread: 16 for LEN in {16, 128, 256} and LEN >= 1024, else 8
-// (LEN <= 8: LEN).
-#ifndef CSG_FFT_E_BIG
-#define CSG_FFT_E_BIG 8
-#endif
-// Lengths 3 * 2^m (384^2 grids) use 24 elements per thread and one radix-3
-// stage first.
+// Elements per thread E (the register tile; the largest radix is the largest
+// power of two <= 16 dividing E).  Default: 16 for LEN in {16, 128, 256}, 8
+// otherwise (LEN <= 8: LEN); lengths 3 * 2^m use 24 and one radix-3 stage
+// first.  Kernels may pick another E per pass (any divisor of LEN, a multiple
+// of 3 for 3 * 2^m).
 constexpr int fft_E(int len) {
   return (len % 3 == 0) ? (len < 24 ? len : 24)
-         : len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : (len >= 1024 ? CSG_FFT_E_BIG : 8));
+         : len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : 8);
+}
+constexpr bool fft_E_valid(int len, int e) {
+  return e >= 1 && e <= len && len % e == 0 && (len % 3 != 0 || e % 3 == 0) && e <= 32;
 }
 // Radix of forward stage s: 3 while 3 divides what remains, then the largest
 // power of two <= 16 dividing both E and what remains.
-constexpr int fft_radix(int len, int s) {
-  const int e = fft_E(len);
+constexpr int fft_radix(int len, int s, int e) {
   int rem = len;
   int r = 1;
   for (int k = 0; k <= s; ++k) {
@@ -178,22 +178,22 @@ constexpr int fft_radix(int len, int s) {
   }
   return r;
 }
-constexpr int fft_nstages(int len) {
+constexpr int fft_nstages(int len, int e) {
   int n = 0, rem = len;
   while (rem > 1) {
-    rem /= fft_radix(len, n);
+    rem /= fft_radix(len, n, e);
     ++n;
   }
   return n;
 }
 // radix / stride product of stage s of the forward (INV = false) or inverse
 // (reversed) plan
-constexpr int plan_radix(int len, int s, bool inv) {
-  return inv ? fft_radix(len, fft_nstages(len) - 1 - s) : fft_radix(len, s);
+constexpr int plan_radix(int len, int s, bool inv, int e) {
+  return inv ? fft_radix(len, fft_nstages(len, e) - 1 - s, e) : fft_radix(len, s, e);
 }
-constexpr int plan_pstride(int len, int s, bool inv) {
+constexpr int plan_pstride(int len, int s, bool inv, int e) {
   int p = 1;
-  for (int k = 0; k < s; ++k) p *= plan_radix(len, k, inv);
+  for (int k = 0; k < s; ++k) p *= plan_radix(len, k, inv, e);
   return p;
 }
 
@@ -224,9 +224,21 @@ CSG_HD cplx<T> tw_at(const cplx<T>* tw, int e) {
   return w;
 }
 
+// Stage the half table in LDS: all loads issued, then all LDS stores.
 template <int NTAB, int NT, class T>
 __device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __restrict__ g) {
-  for (int m = threadIdx.x; m < NTAB / 2; m += NT) lds_tw[m] = g[m];
+  constexpr int K = (NTAB / 2 + NT - 1) / NT;
+  cplx<T> w[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int m = threadIdx.x + NT * r;
+    if (m < NTAB / 2) w[r] = g[m];
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int m = threadIdx.x + NT * r;
+    if (m < NTAB / 2) lds_tw[m] = w[r];
+  }
 }
 
 // Order LDS traffic between stages: lines that fit in one wavefront only need
@@ -279,31 +291,30 @@ __device__ __forceinline__ void stage_exchange(cplx<T> (&v)[E], int t, cplx<T>* 
   lds_sync<WAVE>();
 }
 
-template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE>
-__device__ __forceinline__ void fft_stages(cplx<T> (&v)[NA][fft_E(LEN)], int t, cplx<T>* line,
-                                           const cplx<T>* tw) {
+template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE, int E>
+__device__ __forceinline__ void fft_stages(cplx<T> (&v)[NA][E], int t, cplx<T>* line, const cplx<T>* tw) {
   constexpr bool INV = DIR > 0;
-  constexpr int NS = fft_nstages(LEN);
+  constexpr int NS = fft_nstages(LEN, E);
   if constexpr (S < NS) {
-    constexpr int E = fft_E(LEN), TPL = LEN / E;
-    constexpr int R = plan_radix(LEN, S, INV), P = plan_pstride(LEN, S, INV);
+    constexpr int TPL = LEN / E;
+    constexpr int R = plan_radix(LEN, S, INV, E), P = plan_pstride(LEN, S, INV, E);
 #pragma unroll
     for (int a = 0; a < NA; ++a) stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB>(v[a], t, tw);
     if constexpr (S + 1 < NS) {
 #pragma unroll
       for (int a = 0; a < NA; ++a) stage_exchange<T, LEN, TPL, E, R, P, WAVE>(v[a], t, line);
     }
-    fft_stages<T, DIR, LEN, NA, S + 1, NTAB, WAVE>(v, t, line, tw);
+    fft_stages<T, DIR, LEN, NA, S + 1, NTAB, WAVE, E>(v, t, line, tw);
   }
 }
 
-// Transform NA lines held in canonical registers by this thread group (TPL
-// threads, t = index within the group), using `line` (padded_len<T>(LEN)
+// Transform NA lines held in canonical registers by this thread group (TPL =
+// LEN/E threads, t = index within the group), using `line` (padded_len<T>(LEN)
 // elements of LDS private to the group) for the exchanges.
-template <class T, int DIR, int LEN, int NA, int NTAB, bool WAVE>
-__device__ __forceinline__ void fft_regs(cplx<T> (&v)[NA][fft_E(LEN)], int t, cplx<T>* line,
-                                         const cplx<T>* tw) {
-  fft_stages<T, DIR, LEN, NA, 0, NTAB, WAVE>(v, t, line, tw);
+template <class T, int DIR, int LEN, int NA, int NTAB, bool WAVE, int E = fft_E(LEN)>
+__device__ __forceinline__ void fft_regs(cplx<T> (&v)[NA][E], int t, cplx<T>* line, const cplx<T>* tw) {
+  static_assert(fft_E_valid(LEN, E), "E must divide LEN (a multiple of 3 for 3 * 2^m lines)");
+  fft_stages<T, DIR, LEN, NA, 0, NTAB, WAVE, E>(v, t, line, tw);
 }
 
 // ---- workgroup reductions (wave shuffles, one barrier) ---------------------

@@ -22,8 +22,10 @@ int fail(int code, const std::string& msg);
 
 template <class T>
 bool ops_for(int n, Ops* o) {
-  if constexpr (std::is_same<T, double>::value) return ops_fp64_pow2(n, o) || ops_fp64_pow3(n, o);
-  else return ops_mixed_pow2(n, o) || ops_mixed_pow3(n, o);
+  if constexpr (std::is_same<T, double>::value)
+    return ops_fp64_small(n, o) || ops_fp64_mid(n, o) || ops_fp64_big(n, o) || ops_fp64_pow3(n, o);
+  else
+    return ops_mixed_small(n, o) || ops_mixed_mid(n, o) || ops_mixed_big(n, o) || ops_mixed_pow3(n, o);
 }
 
 inline std::string unsupported_size_message(int n) {

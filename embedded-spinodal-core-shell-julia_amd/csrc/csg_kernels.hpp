@@ -197,9 +197,9 @@ __device__ void finalize_block(const FinalArgs& f, double* scratch) {
 #ifndef CSG_ROW_LDS
 #define CSG_ROW_LDS (40 * 1024)
 #endif
-template <int N, class CT, int NARR>
+template <int N, class CT, int NARR, int E = fft_E(N / 2)>
 constexpr int row_rows() {
-  constexpr int L = N / 2, TPL = L / fft_E(L);
+  constexpr int L = N / 2, TPL = L / E;
   const int lp = NARR * padded_len<CT>(L) * 2 * (int)sizeof(CT);
   int b = 1;
   while (b < 8 && N % (b * 2) == 0 && (b * 2) * lp <= CSG_ROW_LDS && (b * 2) * TPL <= 1024) b *= 2;
@@ -213,26 +213,41 @@ constexpr int row_rows() {
 #define CSG_P3_SPLIT 1
 #endif
 
-template <class CT, class TIN, class TOUT, int N, int MODE>
+// Schedule of a pass kernel (0 = the default rule):
+//   E   elements per thread (register tile; sets the radix plan),
+//   W   rows (row pass) or lines (column pass) per workgroup,
+//   G   row passes: transposed-tile loads issued in groups of G, each group
+//       written to LDS before the next is issued (E = all at once),
+//   PF  1: the pass's other inputs (phi in P1/P3, the state in P4) are loaded
+//       with the first group of loads; 0: where they are used.
+template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1>
+struct Sched {
+  static constexpr int E = E_, W = W_, G = G_, PF = PF_;
+};
+
+template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
 struct RowCfg {
   static constexpr int L = N / 2;
-  static constexpr int E = fft_E(L);
+  static constexpr int E = SC::E ? SC::E : fft_E(L);
   static constexpr int TPL = L / E;
   static constexpr bool SPLIT = (MODE == ROW_P3) && CSG_P3_SPLIT;
   static constexpr int NARR = (MODE == ROW_P3 && !SPLIT) ? 2 : 1;   // arrays per workgroup
-  static constexpr int B = row_rows<N, CT, NARR>();
+  static constexpr int B = SC::W ? SC::W : row_rows<N, CT, NARR, E>();
+  static constexpr int G = SC::G ? SC::G : E;
+  static constexpr int PF = SC::PF;
   static constexpr int LP = padded_len<CT>(L);
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int TILES = N / B;
   static constexpr int GRID = SPLIT ? 2 * TILES : TILES;
-  static_assert(NT % 64 == 0 || NT < 64, "workgroup of whole wavefronts");
+  static_assert(fft_E_valid(L, E) && E % G == 0 && N % B == 0, "row schedule");
+  static_assert(NT <= 1024 && (NT % 64 == 0 || (NT < 64 && (NT & (NT - 1)) == 0)), "workgroup of whole wavefronts");
 };
 
-template <class CT, class TIN, class TOUT, int N, int MODE>
-__global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE>::NT))
+template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
+__global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE, SC>::NT))
 row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
-  using Cfg = RowCfg<CT, TIN, TOUT, N, MODE>;
+  using Cfg = RowCfg<CT, TIN, TOUT, N, MODE, SC>;
   const PassArgs a = instance_view(a0);
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr int NARR = Cfg::NARR;
@@ -251,7 +266,21 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   cplx<CT> v[NARR][E];
   double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // partials, reduced at the end
 
-  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+  // Loads are issued before the first wait where the schedule says so (tile
+  // group, phi, twiddles): one memory latency instead of one per phase.
+  static_assert(B * L == NT * E, "each thread moves E elements of the tile");
+  constexpr int G = Cfg::G;
+  constexpr bool PF = Cfg::PF != 0 && (MODE == ROW_P1 || MODE == ROW_P3);
+  CT ph[E][2];   // phi at this thread's sites (P1, P3)
+  auto load_phi = [&]() {
+    const CT* phi = static_cast<const CT*>(a.phi);
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int k = t + TPL * j;
+      ph[j][0] = phi[rbase + 2 * k];
+      ph[j][1] = phi[rbase + 2 * k + 1];
+    }
+  };
   if constexpr (MODE == ROW_FWD) {
     const double* x = static_cast<const double*>(a.in0);
 #pragma unroll
@@ -259,16 +288,38 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       const int k = t + TPL * j;
       v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
     }
+    if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
     __syncthreads();
   } else {
-    // transposed tile load from line layout: lane-consecutive rows (coalesced)
+    // transposed tile load from line layout: lane-consecutive rows (coalesced),
+    // in groups of G elements per thread
 #pragma unroll
-    for (int q = 0; q < NARR; ++q) {
-      const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
-      for (int f = threadIdx.x; f < B * L; f += NT) {
-        const int bb = f % B, kx = f / B;
-        lds[(q * B + bb) * LP + pad<CT>(kx)] = cvt<CT>(X[(size_t)kx * N + j0 + bb]);
+    for (int g0 = 0; g0 < E; g0 += G) {
+      cplx<TIN> tl[NARR][G];
+#pragma unroll
+      for (int q = 0; q < NARR; ++q) {
+        const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+          const int f = threadIdx.x + NT * (g0 + r);
+          tl[q][r] = X[(size_t)(f / B) * N + j0 + f % B];
+        }
       }
+      if (g0 == 0) {
+        if constexpr (MODE == ROW_P1) {
+          // mean composition from the FP64 k_x = 0 row sums of the state
+          if (threadIdx.x < B) red[1] = static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x);
+        }
+        if constexpr (PF) load_phi();
+        if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+      }
+#pragma unroll
+      for (int q = 0; q < NARR; ++q)
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+          const int f = threadIdx.x + NT * (g0 + r);
+          lds[(q * B + f % B) * LP + pad<CT>(f / B)] = cvt<CT>(tl[q][r]);
+        }
     }
     __syncthreads();
     // C2R pre-processing fused with the read: Z_k = (X_k + conj X_{L-k}) +
@@ -290,7 +341,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     lds_sync<WAVE>();
-    fft_regs<CT, +1, L, NARR, N, WAVE>(v, t, row, tw);
+    fft_regs<CT, +1, L, NARR, N, WAVE, E>(v, t, row, tw);
   }
 
   if constexpr (MODE == ROW_INV) {
@@ -304,17 +355,16 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     return;
   }
 
+  if constexpr ((MODE == ROW_P1 || MODE == ROW_P3) && !PF) load_phi();
   if constexpr (MODE == ROW_P1) {
     // pointwise: f'(c); bulk energy, min, max, non-finite partials
-    const CT* phi = static_cast<const CT*>(a.phi);
     const DevParams& p = a.p;
     const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
     double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      const int k = t + TPL * j;
       const CT cin[2] = {v[0][j].x, v[0][j].y};
-      const CT qq[2] = {phi[rbase + 2 * k], phi[rbase + 2 * k + 1]};
+      const CT qq[2] = {ph[j][0], ph[j][1]};
       CT out[2];
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
@@ -345,13 +395,11 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 
   if constexpr (MODE == ROW_P3) {
     // J = M0 h(phi) g for the gradient component(s); dissipation partial
-    const CT* phi = static_cast<const CT*>(a.phi);
     const CT mob = a.p.mobility;
     double dsum = 0.0;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      const int k = t + TPL * j;
-      const CT q0 = phi[rbase + 2 * k], q1 = phi[rbase + 2 * k + 1];
+      const CT q0 = ph[j][0], q1 = ph[j][1];
       const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
       const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
 #pragma unroll
@@ -366,7 +414,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   }
 
   // R2C along i: forward FFT, then X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k})
-  fft_regs<CT, -1, L, NARR, N, WAVE>(v, t, row, tw);
+  fft_regs<CT, -1, L, NARR, N, WAVE, E>(v, t, row, tw);
 #pragma unroll
   for (int q = 0; q < NARR; ++q) {
     cplx<CT>* tr = lds + (q * B + b) * LP;
@@ -403,15 +451,15 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 #pragma unroll
   for (int q = 0; q < NARR; ++q) {
     cplx<TOUT>* X = static_cast<cplx<TOUT>*>((arr0 + q) == 0 ? a.out0 : a.out1);
-    for (int f = threadIdx.x; f < B * L; f += NT) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const int f = threadIdx.x + NT * r;
       const int bb = f % B, kx = f / B;
       X[(size_t)kx * N + j0 + bb] = cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT>(kx)]);
     }
   }
 
   if constexpr (MODE == ROW_P1) {
-    // mean composition from the FP64 k_x = 0 row sums of the state
-    if (threadIdx.x < B) red[1] = static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x);
     const int ops[5] = {RED_SUM, RED_SUM, RED_MIN, RED_MAX, RED_SUM};
     block_reduce<NT, 5>(red, ops, scratch);
     if (threadIdx.x == 0) {
@@ -431,9 +479,9 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 // ---------------------------------------------------------------------------
 // Column pass: C lines kx0 .. kx0+C-1 of length N per workgroup, TPL threads
 // per line; each thread group transforms both input arrays of its line.
-template <int N, class CT>
+template <int N, class CT, int E = fft_E(N)>
 constexpr int col_lines() {
-  constexpr int TPL = N / fft_E(N);
+  constexpr int TPL = N / E;
   if (TPL > 64) return 1;   // lines spanning wavefronts: one per workgroup
   const int lp = padded_len<CT>(N) * 2 * (int)sizeof(CT);
   int c = 1;
@@ -442,22 +490,25 @@ constexpr int col_lines() {
   return c;
 }
 
-template <class CT, class TS, int N, int MODE>
+template <class CT, class TS, int N, int MODE, class SC = Sched<>>
 struct ColCfg {
-  static constexpr int E = fft_E(N);
+  static constexpr int E = SC::E ? SC::E : fft_E(N);
   static constexpr int TPL = N / E;
-  static constexpr int C = col_lines<N, CT>();
+  static constexpr int C = SC::W ? SC::W : col_lines<N, CT, E>();
+  static constexpr int PF = SC::PF;
   static constexpr int LP = padded_len<CT>(N);
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = (N / 2) / C;
+  static_assert(fft_E_valid(N, E) && (N / 2) % C == 0, "column schedule");
   static_assert(WAVE || C == 1, "block-synchronised lines need one line per workgroup");
+  static_assert(NT <= 1024 && (NT % 64 == 0 || (NT < 64 && (NT & (NT - 1)) == 0)), "workgroup of whole wavefronts");
 };
 
-template <class CT, class TS, int N, int MODE>
-__global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE>::NT))
+template <class CT, class TS, int N, int MODE, class SC = Sched<>>
+__global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE, SC>::NT))
 col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
-  using Cfg = ColCfg<CT, TS, N, MODE>;
+  using Cfg = ColCfg<CT, TS, N, MODE, SC>;
   const PassArgs a = instance_view(a0);
   constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
@@ -473,8 +524,8 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   const double inv_nsp = p.inv_nsp;
   const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);  // (pi/spacing)^2
 
-  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
   cplx<CT> v[2][E];
+  cplx<TS> sp[E];   // P4: the state line, loaded with the inputs
   if constexpr (MODE == COL_P2 || MODE == COL_MU) {
     const cplx<TS>* S = static_cast<const cplx<TS>*>(a.in0);
     const cplx<CT>* F = static_cast<const cplx<CT>*>(a.in1);
@@ -491,9 +542,16 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       v[0][j] = X0[base + t + TPL * j];
       v[1][j] = X1[base + t + TPL * j];
     }
+    if constexpr (MODE == COL_P4 && Cfg::PF) {
+      const cplx<TS>* S = static_cast<const cplx<TS>*>(a.out0);
+#pragma unroll
+      for (int j = 0; j < E; ++j) sp[j] = S[base + t + TPL * j];
+    }
   }
+  // twiddles after the line loads: a single wait covers both
+  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<CT, -1, N, 2, N, WAVE>(v, t, line, tw);
+  fft_regs<CT, -1, N, 2, N, WAVE, E>(v, t, line, tw);
 
   // packed line 0 (k_x = 0 in re, k_x = N/2 in im) needs the value at -ky of
   // one array: stage it through the line buffer
@@ -545,7 +603,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     if constexpr (MODE == COL_P2) {
-      fft_regs<CT, +1, N, 2, N, WAVE>(v, t, line, tw);
+      fft_regs<CT, +1, N, 2, N, WAVE, E>(v, t, line, tw);
       cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
       cplx<CT>* O1 = static_cast<cplx<CT>*>(a.out1);
 #pragma unroll
@@ -561,7 +619,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       cplx<CT> w[1][E];
 #pragma unroll
       for (int j = 0; j < E; ++j) w[0][j] = v[0][j];
-      fft_regs<CT, +1, N, 1, N, WAVE>(w, t, line, tw);
+      fft_regs<CT, +1, N, 1, N, WAVE, E>(w, t, line, tw);
       cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
 #pragma unroll
       for (int j = 0; j < E; ++j) O0[base + t + TPL * j] = w[0][j];
@@ -613,12 +671,16 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     if constexpr (MODE == COL_P4) {
-      fft_regs<CT, +1, N, 1, N, WAVE>(w, t, line, tw);
+      fft_regs<CT, +1, N, 1, N, WAVE, E>(w, t, line, tw);
       // S += IFFT_j(inc)
       cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
+      if constexpr (!Cfg::PF) {
+#pragma unroll
+        for (int j = 0; j < E; ++j) sp[j] = S[base + t + TPL * j];
+      }
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        cplx<TS> s = S[base + t + TPL * j];
+        cplx<TS> s = sp[j];
         s.x += static_cast<TS>(w[0][j].x);
         s.y += static_cast<TS>(w[0][j].y);
         S[base + t + TPL * j] = s;

@@ -1,0 +1,14 @@
+// Kernel instantiations: FP64, N in {2048, 4096}.
+#include "csg_ops.hpp"
+
+namespace csg {
+
+bool ops_fp64_big(int n, Ops* o) {
+  switch (n) {
+    case 2048: *o = make_ops<2048, double>(); return true;
+    case 4096: *o = make_ops<4096, double>(); return true;
+    default: return false;
+  }
+}
+
+}  // namespace csg

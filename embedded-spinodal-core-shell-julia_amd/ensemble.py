@@ -130,6 +130,22 @@ class Ensemble:
     def device_bytes(self) -> int:
         return int(lib().csg_batch_device_bytes(self.handle))
 
+    def schedule(self, n: int) -> tuple:
+        """Selected kernel candidates of the members of size n (see Session)."""
+        sel = (ctypes.c_int32 * 4)()
+        check(lib().csg_batch_get_schedule(self.handle, int(n), sel))
+        return tuple(sel)
+
+    def set_schedule(self, n: int, sel):
+        if len(sel) != 4:
+            raise ValueError("one candidate per pass")
+        check(lib().csg_batch_set_schedule(self.handle, int(n), (ctypes.c_int32 * 4)(*(int(x) for x in sel))))
+
+    def autotune(self, dt: float = 0.01, rounds: int = 1) -> dict:
+        """Tune each size class alone (resident fields kept): {n: selection}."""
+        check(lib().csg_batch_autotune(self.handle, float(dt), int(rounds)))
+        return {n: self.schedule(n) for n in sorted(set(self.ns))}
+
 
 def run_ensemble(c0s: Sequence, phis: Sequence, dt: float = 0.01, steps: int = 1000,
                  spacings: Optional[Sequence[float]] = None, params: Optional[Sequence[Parameters]] = None, *,

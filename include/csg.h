@@ -144,6 +144,28 @@ int csg_adjoint_backward(csg_ctx* ctx, const double* dL_dcT, double* dL_dc0, dou
 /* Bytes of device memory held by the context. */
 int64_t csg_device_bytes(const csg_ctx* ctx);
 
+/* ---- kernel schedules ------------------------------------------------------
+ * No counterpart in the reference (FFTW plans its transforms itself).  Each
+ * of the four pass kernels of a step (pass 0..3 = P1 row, P2 column, P3 row,
+ * P4 column) is compiled in up to 3 candidate schedules, fastest first as
+ * measured on MI355X; candidate 0 is the default.  All candidates compute the
+ * same step (they differ in the FFT radix plan, so results agree to rounding,
+ * not bit for bit).
+ *
+ * csg_schedule_info: count of candidates of `pass`; `sched` (may be NULL,
+ * else 4 * count int32) receives each candidate's [elements per thread, rows
+ * or lines per workgroup, tile-load group, prefetch]. */
+int csg_schedule_info(const csg_ctx* ctx, int32_t pass, int32_t* count, int32_t* sched);
+/* Selected candidate per pass (sel[0..3]). */
+int csg_get_schedule(const csg_ctx* ctx, int32_t* sel);
+int csg_set_schedule(csg_ctx* ctx, const int32_t* sel);
+/* Runtime tuner: times the captured loop at `dt` for the candidates (rounds
+ * of coordinate descent over the passes, starting from the current
+ * selection), keeps the fastest combination and returns it in sel_out (may be
+ * NULL) with its device time per loop iteration in microseconds (may be
+ * NULL).  Needs csg_set_fields; the resident field is restored. */
+int csg_autotune(csg_ctx* ctx, double dt, int32_t rounds, int32_t* sel_out, double* us_per_step);
+
 /* ---- ensembles ------------------------------------------------------------
  * No counterpart in the reference, which runs one particle per call: an
  * ensemble is `count` independent instances of run_simulation
@@ -177,6 +199,13 @@ int64_t csg_batch_nonfinite_step(const csg_batch* b, int32_t i);
 
 /* Bytes of device memory held by the ensemble. */
 int64_t csg_batch_device_bytes(const csg_batch* b);
+
+/* Schedules of the members of grid size n (as csg_get_schedule /
+ * csg_set_schedule), and the runtime tuner applied to each size class alone
+ * (the resident fields are restored). */
+int csg_batch_get_schedule(const csg_batch* b, int32_t n, int32_t* sel);
+int csg_batch_set_schedule(csg_batch* b, int32_t n, const int32_t* sel);
+int csg_batch_autotune(csg_batch* b, double dt, int32_t rounds);
 
 const char* csg_last_error(void);
 

@@ -11,13 +11,13 @@
 
 using namespace csg;
 
-template <int DIR, int LEN, int S, class T, int NTAB>
+template <int DIR, int LEN, int S, class T, int NTAB, int E>
 void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::vector<cplx<T>>& line) {
   constexpr bool INV = DIR > 0;
-  constexpr int NS = fft_nstages(LEN);
+  constexpr int NS = fft_nstages(LEN, E);
   if constexpr (S < NS) {
-    constexpr int E = fft_E(LEN), TPL = LEN / E;
-    constexpr int R = plan_radix(LEN, S, INV), P = plan_pstride(LEN, S, INV);
+    constexpr int TPL = LEN / E;
+    constexpr int R = plan_radix(LEN, S, INV, E), P = plan_pstride(LEN, S, INV, E);
     static_assert(E % R == 0, "radix must divide E");
     for (int t = 0; t < TPL; ++t) {
       cplx<T> r[E];
@@ -33,15 +33,16 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
       for (int t = 0; t < TPL; ++t)
         for (int j = 0; j < E; ++j) v[t][j] = line[pad<T>(t + TPL * j)];
     }
-    run_stages<DIR, LEN, S + 1, T, NTAB>(v, tw, line);
+    run_stages<DIR, LEN, S + 1, T, NTAB, E>(v, tw, line);
   }
 }
 
 // NTAB: length of the twiddle table's full circle (LEN for column lines,
 // 2 LEN for the half-length complex transforms of the real row passes).
-template <int LEN, class T, int NTAB = LEN>
+template <int LEN, class T, int NTAB = LEN, int E = fft_E(LEN)>
 int check(double tol) {
-  constexpr int E = fft_E(LEN), TPL = LEN / E;
+  static_assert(fft_E_valid(LEN, E), "E");
+  constexpr int TPL = LEN / E;
   std::vector<cplx<T>> tw(NTAB / 2);
   for (int m = 0; m < NTAB / 2; ++m) {
     const long double a = -2.0L * 3.141592653589793238462643383279502884L * m / NTAB;
@@ -65,8 +66,8 @@ int check(double tol) {
         v[t][j] = cplx<T>{static_cast<T>(z.real()), static_cast<T>(z.imag())};
       }
     std::vector<cplx<T>> line(padded_len<T>(LEN));
-    if (dir < 0) run_stages<-1, LEN, 0, T, NTAB>(v, tw.data(), line);
-    else run_stages<+1, LEN, 0, T, NTAB>(v, tw.data(), line);
+    if (dir < 0) run_stages<-1, LEN, 0, T, NTAB, E>(v, tw.data(), line);
+    else run_stages<+1, LEN, 0, T, NTAB, E>(v, tw.data(), line);
     double err = 0, mag = 0;
     for (int k = 0; k < LEN; ++k) {
       std::complex<long double> acc = 0;
@@ -79,7 +80,7 @@ int check(double tol) {
       mag = std::fmax(mag, std::abs(std::complex<double>((double)acc.real(), (double)acc.imag())));
     }
     const bool ok = err <= tol * mag;
-    printf("LEN=%5d NTAB=%5d E=%2d stages=%d dir=%+d %s rel err %.3e\n", LEN, NTAB, E, fft_nstages(LEN), dir,
+    printf("LEN=%5d NTAB=%5d E=%2d stages=%d dir=%+d %s rel err %.3e\n", LEN, NTAB, E, fft_nstages(LEN, E), dir,
            sizeof(T) == 8 ? "f64" : "f32", err / mag);
     if (!ok) { printf("  FAIL\n"); ++bad; }
   }
@@ -117,6 +118,18 @@ int main() {
   bad += check<192, double, 384>(1e-14);
   bad += check<384, double, 768>(1e-14);
   bad += check<192, float, 384>(2e-6);
+  // other register tiles (elements per thread) the pass schedules may use
+  bad += check<512, double, 1024, 4>(1e-14);
+  bad += check<1024, double, 1024, 4>(1e-14);
+  bad += check<1024, double, 1024, 16>(1e-14);
+  bad += check<2048, double, 4096, 16>(1e-14);
+  bad += check<256, double, 512, 4>(1e-14);
+  bad += check<128, double, 128, 8>(1e-14);
+  bad += check<256, float, 256, 8>(2e-6);
+  bad += check<192, double, 384, 12>(1e-14);
+  bad += check<384, double, 384, 12>(1e-14);
+  bad += check<384, double, 384, 6>(1e-14);
+  bad += check<96, double, 192, 6>(1e-14);
   printf(bad ? "FAILED %d\n" : "ALL OK\n", bad);
   return bad ? 1 : 0;
 }
