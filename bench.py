@@ -14,6 +14,7 @@ Also reported:
                 N^2 FFTs per iteration), 1 thread, on a bounded sample of the
                 same workload
   ensemble      the uneven 256-particle ensemble (mixed and FP64 forms)
+  large_field   4096^2 Janus-type field, 5000 steps, per-pass HBM GB/s, mixed error
   gradient      gradient evaluations/s of the 512^2 adjoint workload
 """
 from __future__ import annotations
@@ -51,6 +52,7 @@ def _args():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--large-steps", type=int, default=5000, help="steps of the 4096^2 field (0: skip)")
     ap.add_argument("--ens-count", type=int, default=256, help="ensemble members (0: skip the ensemble)")
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--grad-steps", type=int, default=2000,
@@ -129,6 +131,53 @@ def gradient_rate(pkg, a):
                         f"{max(1, round(steps ** 0.5))}", "gradient_evals_per_s": 1.0 / secs,
             "seconds_per_gradient": secs, "loss": g.loss,
             "forward_equivalent_site_updates_per_s": 3 * n * n * steps / secs}
+
+
+# Janus-type two-well set of the large-field workload: B raised 2x and chi
+# shifted by +0.3 from core_shell.jl's defaults, with the stabilisation Q
+# raised 2x: at Q = 0.5 the scheme turns non-finite at dt = 0.01 with B = 8
+# (step 964 at 4096^2).  The reference has no such set; this is the definition
+# used here.
+def janus_params(pkg):
+    d = pkg.Parameters()
+    return pkg.Parameters(B=2.0 * d.B, chi=d.chi + 0.3, Q=2.0 * d.Q)
+
+
+def large_field(pkg, a):
+    """The large single field: 4096^2, R = 1300, w = 2, dt, a.large_steps
+    steps, Janus-type parameters; FP64 and mixed forms on the same inputs, HBM
+    GB/s per pass from event-timed passes, mixed error against FP64."""
+    n = 4096
+    c0, phi = pkg.initial_fields(n, radius=1300.0, width=2.0, seed=5749)
+    p = janus_params(pkg)
+    res = {}
+    for form in ("fp64", "mixed"):
+        s = pkg.Session(n, 1.0, p, form)
+        s.set_fields(c0, phi)
+        s.run(a.dt, 64)                            # captures the step graph
+        s.set_fields(c0, phi)
+        t0 = time.perf_counter()
+        hist = s.run(a.dt, a.large_steps)
+        secs = time.perf_counter() - t0
+        res[form] = dict(secs=secs, hist=hist, c=s.field(), ms=s.time_passes(a.dt, 20), sched=s.schedule)
+        s.close()
+    out = {"workload": f"large field {n}x{n}, R=1300, w=2, Janus-type set B={p.B:g}, chi={p.chi:g}, Q={p.Q:g}, "
+                       f"{a.large_steps} steps, dt={a.dt:g}, history every step"}
+    sites = n * n
+    for form, r in res.items():
+        ms = r["ms"]
+        out[form] = {"site_updates_per_s": sites * (a.large_steps + 1) / r["secs"],
+                     "ms_per_step": 1e3 * r["secs"] / (a.large_steps + 1),
+                     "pass_gbs": {PASS_NAMES[k]: PASS_BYTES[form][k] * sites / (ms[k] * 1e-3) / 1e9 for k in range(4)},
+                     "pass_ms": {PASS_NAMES[k]: float(ms[k]) for k in range(4)},
+                     "step_frac_hbm": sum(PASS_BYTES[form]) * sites * (a.large_steps + 1) / r["secs"] / 1e9 / HBM_PEAK_GBS,
+                     "schedule": list(r["sched"]),
+                     "mass_drift": float(np.max(np.abs(r["hist"][:, 1] - r["hist"][0, 1]))),
+                     "energy_final": float(r["hist"][-1, 2])}
+    out["mixed"]["max_dc_vs_fp64"] = float(np.max(np.abs(res["mixed"]["c"] - res["fp64"]["c"])))
+    out["mixed"]["rel_energy_err_vs_fp64"] = float(np.max(np.abs(res["mixed"]["hist"][:, 2] - res["fp64"]["hist"][:, 2])
+                                                          / np.abs(res["fp64"]["hist"][:, 2])))
+    return out
 
 
 def ensemble_rate(pkg, a):
@@ -263,6 +312,8 @@ def main():
     }
     if a.ens_count:
         line["ensemble"] = ensemble_rate(pkg, a)
+    if a.large_steps:
+        line["large_field"] = large_field(pkg, a)
     if a.grad_steps:
         line["gradient"] = gradient_rate(pkg, a)
     if not a.no_cpu:

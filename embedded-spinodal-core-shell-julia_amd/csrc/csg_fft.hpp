@@ -152,11 +152,11 @@ template <int R, int DIR, int ST, class T> CSG_HD void dft(cplx<T>* x) {
 // ---- plans -----------------------------------------------------------------
 // Elements per thread E (the register tile; the largest radix is the largest
 // power of two <= 16 dividing E).  Default: 16 for LEN in {16, 128, 256}, 8
-// otherwise (LEN <= 8: LEN); lengths 3 * 2^m use 24 and one radix-3 stage
+// otherwise (LEN <= 8: LEN); lengths 3 * 2^m use 12 and one radix-3 stage
 // first.  Kernels may pick another E per pass (any divisor of LEN, a multiple
 // of 3 for 3 * 2^m).
 constexpr int fft_E(int len) {
-  return (len % 3 == 0) ? (len < 24 ? len : 24)
+  return (len % 3 == 0) ? (len < 12 ? len : 12)
          : len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : 8);
 }
 constexpr bool fft_E_valid(int len, int e) {
@@ -224,20 +224,22 @@ CSG_HD cplx<T> tw_at(const cplx<T>* tw, int e) {
   return w;
 }
 
-// Stage the half table in LDS: all loads issued, then all LDS stores.
+// Stage the half table in LDS: all loads issued, then all LDS stores.  The
+// loads are unconditional (index clamped into the table) so the staging
+// values stay in registers; only the stores are predicated.
 template <int NTAB, int NT, class T>
 __device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __restrict__ g) {
-  constexpr int K = (NTAB / 2 + NT - 1) / NT;
+  constexpr int H = NTAB / 2, K = (H + NT - 1) / NT;
   cplx<T> w[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
-    const int m = threadIdx.x + NT * r;
-    if (m < NTAB / 2) w[r] = g[m];
+    const int m = static_cast<int>(threadIdx.x) + NT * r;
+    w[r] = g[m < H ? m : H - 1];
   }
 #pragma unroll
   for (int r = 0; r < K; ++r) {
-    const int m = threadIdx.x + NT * r;
-    if (m < NTAB / 2) lds_tw[m] = w[r];
+    const int m = static_cast<int>(threadIdx.x) + NT * r;
+    if (m < H) lds_tw[m] = w[r];
   }
 }
 

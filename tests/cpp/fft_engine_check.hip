@@ -130,6 +130,9 @@ int main() {
   bad += check<384, double, 384, 12>(1e-14);
   bad += check<384, double, 384, 6>(1e-14);
   bad += check<96, double, 192, 6>(1e-14);
+  bad += check<384, double, 768, 24>(1e-14);
+  bad += check<768, double, 768, 24>(1e-14);
+  bad += check<768, float, 768, 24>(2e-6);
   printf(bad ? "FAILED %d\n" : "ALL OK\n", bad);
   return bad ? 1 : 0;
 }
