@@ -13,7 +13,7 @@ using DelimitedFiles
 using Libdl
 
 export Parameters, make_grid, initial_fields, geometry, bulk_energy, bulk_derivative,
-       chemical_potential, transport_rate, step, energy, run_simulation
+       chemical_potential, transport_rate, step, energy, run_simulation, run_ensemble
 
 const libcsg = joinpath(@__DIR__, "..", "libcsg.so")
 
@@ -176,6 +176,65 @@ function run_simulation(c0, phi; dt=0.01, steps=1000, spacing=1.0, p=Parameters(
         writedlm(joinpath(directory, "particle.csv"), phi, ',')
     end
     return c, history
+end
+
+# ---- ensembles (csg_batch_*): run_simulation of many particles at once ----
+# Returns [(c, history), ...]; a non-finite member raises after the run with
+# every member's first non-finite step (-1: finite).
+function run_ensemble(c0s::Vector, phis::Vector; dt=0.01, steps=1000, spacings=nothing,
+                      params=nothing, precision=:mixed, history_every=1, device=0)
+    k = length(c0s)
+    ns = Int32[size(c, 1) for c in c0s]
+    sp = spacings === nothing ? C_NULL : Float64.(spacings)
+    ps = params === nothing ? C_NULL : Parameters[p for p in params]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve ns sp ps check(ccall((:csg_batch_create, libcsg), Cint,
+                                     (Ref{Ptr{Cvoid}}, Int32, Ptr{Int32}, Ptr{Float64}, Ptr{Parameters}, Int32, Int32),
+                                     out, k, ns, sp, ps, precision === :mixed ? 1 : 0, device))
+    b = out[]
+    try
+        for i in 1:k
+            c0m, phim = Matrix{Float64}(c0s[i]), Matrix{Float64}(phis[i])
+            GC.@preserve c0m phim check(ccall((:csg_batch_set_fields, libcsg), Cint,
+                                              (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}), b, i - 1, c0m, phim))
+        end
+        rows = steps ÷ history_every + 1
+        hs = [zeros(rows, 6) for _ in 1:k]
+        ptrs = [pointer(h) for h in hs]
+        rc = GC.@preserve hs ptrs ccall((:csg_batch_run, libcsg), Cint,
+                                        (Ptr{Cvoid}, Float64, Int64, Int64, Ptr{Ptr{Float64}}),
+                                        b, dt, steps, history_every, ptrs)
+        if rc == CSG_ERR_NONFINITE
+            bad = [ccall((:csg_batch_nonfinite_step, libcsg), Int64, (Ptr{Cvoid}, Int32), b, i - 1) for i in 1:k]
+            error("Non-finite composition in ensemble members $(findall(>=(0), bad)) at steps $(filter(>=(0), bad))")
+        end
+        check(rc)
+        res = Tuple{Matrix{Float64}, Matrix{Float64}}[]
+        for i in 1:k
+            c = Matrix{Float64}(undef, ns[i], ns[i])
+            GC.@preserve c check(ccall((:csg_batch_get_field, libcsg), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                                       b, i - 1, c))
+            push!(res, (c, hs[i]))
+        end
+        return res
+    finally
+        ccall((:csg_batch_destroy, libcsg), Cint, (Ptr{Cvoid},), b)
+    end
+end
+
+# ---- kernel schedules: tune a context's step passes (csg_autotune) ----
+# Returns (selection per pass, device microseconds per loop iteration).
+function autotune!(ctx::Context; dt=0.01, rounds=2)
+    sel = zeros(Int32, 4)
+    us = Ref{Float64}(0.0)
+    GC.@preserve sel check(ccall((:csg_autotune, libcsg), Cint, (Ptr{Cvoid}, Float64, Int32, Ptr{Int32}, Ref{Float64}),
+                                 ctx.ptr, dt, rounds, sel, us))
+    return Tuple(sel), us[]
+end
+
+function set_schedule!(ctx::Context, sel)
+    s = Int32.(collect(sel))
+    GC.@preserve s check(ccall((:csg_set_schedule, libcsg), Cint, (Ptr{Cvoid}, Ptr{Int32}), ctx.ptr, s))
 end
 
 end # module
