@@ -32,7 +32,7 @@ EXPORTS = (
     "csg_batch_create", "csg_batch_destroy", "csg_batch_set_fields", "csg_batch_run",
     "csg_batch_get_field", "csg_batch_nonfinite_step", "csg_batch_device_bytes",
     "csg_schedule_info", "csg_get_schedule", "csg_set_schedule", "csg_autotune",
-    "csg_batch_get_schedule", "csg_batch_set_schedule", "csg_batch_autotune",
+    "csg_batch_get_schedule", "csg_batch_set_schedule", "csg_batch_autotune", "csg_set_params",
 )
 
 
@@ -98,6 +98,7 @@ def lib():
         "csg_batch_get_schedule": (ctypes.c_int, [_VP, i32, ctypes.POINTER(i32)]),
         "csg_batch_set_schedule": (ctypes.c_int, [_VP, i32, ctypes.POINTER(i32)]),
         "csg_batch_autotune": (ctypes.c_int, [_VP, dbl, i32]),
+        "csg_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(CsgParams)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

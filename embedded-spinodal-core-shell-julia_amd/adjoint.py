@@ -36,18 +36,36 @@ def _energy(s: Session, c) -> float:
     return float(F[0])
 
 
-def _energy_param_grads(c, phi, spacing, p: Parameters, device: int):
-    """dF/dtheta for theta in (kappa, P, chi): F is affine in each."""
+def _energy_param_grads(s: Session, c, p: Parameters):
+    """dF/dtheta for theta in (kappa, P, chi): F is affine in each, so each is
+    F(theta = 1) - F(theta = 0).  Uses the session with its parameters swapped
+    in place and restores them."""
     out = []
-    for name in ("kappa", "P", "chi"):
-        vals = []
-        for v in (1.0, 0.0):
-            s = Session(c.shape[0], spacing, dataclasses.replace(p, **{name: v}), "fp64", device)
-            s.set_fields(c, phi)
-            vals.append(_energy(s, c))
-            s.close()
-        out.append(vals[0] - vals[1])
+    try:
+        for name in ("kappa", "P", "chi"):
+            vals = []
+            for v in (1.0, 0.0):
+                s.set_params(dataclasses.replace(p, **{name: v}))
+                vals.append(_energy(s, c))
+            out.append(vals[0] - vals[1])
+    finally:
+        s.set_params(p)
     return np.array(out)
+
+
+# One FP64 context per (n, spacing, device), kept between calls: the checkpoint
+# and segment buffers and the captured adjoint graphs are then reused.
+_contexts: dict = {}
+
+
+def _context(n: int, spacing: float, p: Parameters, device: int) -> Session:
+    key = (int(n), float(spacing), int(device))
+    s = _contexts.get(key)
+    if s is None:
+        s = _contexts[key] = Session(n, spacing, p, "fp64", device)
+    elif s.p != p:
+        s.set_params(p)
+    return s
 
 
 def gradient(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float = 0.0,
@@ -56,42 +74,39 @@ def gradient(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float
     c0, phi, target = _fortran(c0), _fortran(phi), _fortran(target)
     n = c0.shape[0]
     K = ckpt_every or max(1, int(round(math.sqrt(steps))))
-    s = Session(n, spacing, p, "fp64", device)
-    try:
-        s.set_fields(c0, phi)
-        check(lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K)))
-        cT = s.field()
-        L = 0.5 * float(np.sum((cT - target) ** 2))
-        gbar = cT - target
-        dtheta = np.zeros(3)
-        if lam != 0.0:
-            L += lam * _energy(s, cT)
-            mu = np.empty((n, n), order="F")
-            check(lib().csg_chemical_potential(s.handle, ptr(_fortran(cT)), ptr(mu)))
-            gbar = gbar + lam * spacing ** 2 * mu
-            dtheta += lam * _energy_param_grads(cT, phi, spacing, p, device)
-        gbar = _fortran(gbar)
-        dc0 = np.empty((n, n), order="F")
-        dp = np.zeros(3)
-        check(lib().csg_adjoint_backward(s.handle, ptr(gbar), ptr(dc0), ptr(dp)))
-        dtheta += dp
-    finally:
-        s.close()
+    s = _context(n, spacing, p, device)
+    s.set_fields(c0, phi)
+    check(lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K)))
+    cT = s.field()
+    L = 0.5 * float(np.sum((cT - target) ** 2))
+    gbar = cT - target
+    dtheta = np.zeros(3)
+    if lam != 0.0:
+        L += lam * _energy(s, cT)
+        mu = np.empty((n, n), order="F")
+        check(lib().csg_chemical_potential(s.handle, ptr(_fortran(cT)), ptr(mu)))
+        gbar = gbar + lam * spacing ** 2 * mu
+    gbar = _fortran(gbar)
+    dc0 = np.empty((n, n), order="F")
+    dp = np.zeros(3)
+    check(lib().csg_adjoint_backward(s.handle, ptr(gbar), ptr(dc0), ptr(dp)))
+    dtheta += dp
+    if lam != 0.0:
+        # after the backward sweep: swapping parameters discards the checkpoints
+        dtheta += lam * _energy_param_grads(s, cT, p)
     return Gradient(L, dc0, float(dtheta[0]), float(dtheta[1]), float(dtheta[2]), cT)
 
 
 def loss(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float = 0.0,
          spacing: float = 1.0, p: Parameters = Parameters(), device: int = 0) -> float:
-    """The forward loss alone (for finite-difference checks)."""
+    """The forward loss alone (for finite-difference checks); the same kernels
+    as the adjoint's forward run, through the captured loop."""
     c0, phi = _fortran(c0), _fortran(phi)
-    s = Session(c0.shape[0], spacing, p, "fp64", device)
-    try:
-        s.set_fields(c0, phi)
-        s.step(dt, steps)
-        cT = s.field()
-        val = 0.5 * float(np.sum((cT - np.asarray(target)) ** 2))
-        if lam != 0.0:
-            val += lam * _energy(s, cT)
-    finally:
-        s.close()
+    s = _context(c0.shape[0], spacing, p, device)
+    s.set_fields(c0, phi)
+    s.run(dt, steps, history_every=max(1, steps))
+    cT = s.field()
+    val = 0.5 * float(np.sum((cT - np.asarray(target)) ** 2))
+    if lam != 0.0:
+        val += lam * _energy(s, cT)
     return val

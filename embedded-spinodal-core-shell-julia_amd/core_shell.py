@@ -152,6 +152,11 @@ class Session:
     def set_schedule(self, sel):
         check(lib().csg_set_schedule(self.handle, _sel4(sel)))
 
+    def set_params(self, p: Parameters):
+        """Replace the parameters in place (no reallocation; field kept)."""
+        check(lib().csg_set_params(self.handle, p._c()))
+        self.p = p
+
     def autotune(self, dt: float = 0.01, rounds: int = 2):
         """Time the candidates inside the captured loop (needs set_fields; the
         resident field is kept): returns (selection, device us per step)."""

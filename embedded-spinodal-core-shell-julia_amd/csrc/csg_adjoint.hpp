@@ -38,11 +38,16 @@ struct AdjArgs {
   DevParams p;
 };
 
+// Register tile of the adjoint passes: 4 elements per thread for power-of-two
+// lines of 64 .. 1024 (the fastest tile of the forward passes there), the
+// default otherwise.
+constexpr int adj_E(int len) { return (len % 3 != 0 && len >= 64 && len <= 1024) ? 4 : fft_E(len); }
+
 template <int N>
 struct AdjColCfg {
-  static constexpr int E = fft_E(N);
+  static constexpr int E = adj_E(N);
   static constexpr int TPL = N / E;
-  static constexpr int C = col_lines<N, double>();
+  static constexpr int C = col_lines<N, double, E>();
   static constexpr int LP = padded_len<double>(N);
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
@@ -89,7 +94,6 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);
   const double q2 = 2.0 * p.Q * p.kappa;
 
-  if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
   cplx<double> v[2][E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -97,8 +101,9 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     v[0][j] = a.G[o];
     v[1][j] = a.S1[o] - a.S0[o];
   }
+  if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<double, -1, N, 2, N, WAVE>(v, t, line, tw);
+  fft_regs<double, -1, N, 2, N, WAVE, E>(v, t, line, tw);
   cplx<double> gn[E], dn[E];
   if (kx == 0) {
     partner_of<N, E, TPL, WAVE>(v[0], gn, t, line);
@@ -131,7 +136,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     v[0][j] = mul_i(sx * V);
     v[1][j] = mul_i(sy * V);
   }
-  fft_regs<double, +1, N, 2, N, WAVE>(v, t, line, tw);
+  fft_regs<double, +1, N, 2, N, WAVE, E>(v, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     a.X0[base + t + TPL * j] = v[0][j];
@@ -162,17 +167,18 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);
   const double tk = 2.0 * p.kappa;
 
-  if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
-  cplx<double> v[3][E];
+  cplx<double> v[3][E], gp[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const size_t o = base + t + TPL * j;
     v[0][j] = a.X0[o];
     v[1][j] = a.X1[o];
     v[2][j] = a.S0[o];
+    gp[j] = a.G[o];   // the adjoint line, updated at the end
   }
+  if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<double, -1, N, 3, N, WAVE>(v, t, line, tw);
+  fft_regs<double, -1, N, 3, N, WAVE, E>(v, t, line, tw);
   cplx<double> jyn[E], cn[E];
   if (kx == 0) {
     partner_of<N, E, TPL, WAVE>(v[1], jyn, t, line);
@@ -207,12 +213,12 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     w[0][j] = (1.0 / N) * r;
     w[1][j] = (dt / N) * lap;
   }
-  fft_regs<double, +1, N, 2, N, WAVE>(w, t, line, tw);
+  fft_regs<double, +1, N, 2, N, WAVE, E>(w, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const size_t o = base + t + TPL * j;
     a.Xr[o] = w[0][j];
-    a.G[o] = a.G[o] + w[1][j];
+    a.G[o] = gp[j] + w[1][j];
   }
   double rr[1] = {acc};
   const int ops[1] = {RED_SUM};
@@ -223,9 +229,9 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
 template <int N>
 struct AdjRowCfg {
   static constexpr int L = N / 2;
-  static constexpr int E = fft_E(L);
+  static constexpr int E = adj_E(L);
   static constexpr int TPL = L / E;
-  static constexpr int B = row_rows<N, double, 2>();
+  static constexpr int B = row_rows<N, double, 2, E>();
   static constexpr int LP = padded_len<double>(L);
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;
@@ -249,12 +255,31 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   const size_t rbase = (size_t)(j0 + b) * N;
   const DevParams& p = a.p;
 
+  // every global load first: transposed tiles of r_v and the state, the
+  // adjoint tile updated at the end, phi, twiddles
+  static_assert(B * L == NT * E, "each thread moves E elements of the tile");
+  cplx<double> tr_x[E], tr_s[E], gp[E];
+  double ph[E][2];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int f = threadIdx.x + NT * r;
+    const size_t o = (size_t)(f / B) * N + j0 + f % B;
+    tr_x[r] = a.Xr[o];
+    tr_s[r] = a.S0[o];
+    gp[r] = a.G[o];
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int k = t + TPL * j;
+    ph[j][0] = a.phi[rbase + 2 * k];
+    ph[j][1] = a.phi[rbase + 2 * k + 1];
+  }
   if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
-  // transposed loads of r_v and the state
-  for (int f = threadIdx.x; f < B * L; f += NT) {
-    const int bb = f % B, kx = f / B;
-    lds[bb * LP + pad<double>(kx)] = a.Xr[(size_t)kx * N + j0 + bb];
-    lds[(B + bb) * LP + pad<double>(kx)] = a.S0[(size_t)kx * N + j0 + bb];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int f = threadIdx.x + NT * r;
+    lds[(f % B) * LP + pad<double>(f / B)] = tr_x[r];
+    lds[(B + f % B) * LP + pad<double>(f / B)] = tr_s[r];
   }
   __syncthreads();
   cplx<double> v[2][E];
@@ -275,20 +300,19 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     }
   }
   lds_sync<WAVE>();
-  fft_regs<double, +1, L, 2, N, WAVE>(v, t, row, tw);
+  fft_regs<double, +1, L, 2, N, WAVE, E>(v, t, row, tw);
   // w = dt f''(c) r_v (core_shell.jl:53-54 differentiated in c); <r_v, g>
   const double A = p.A, Bc = p.B, cb = p.cb, cc = p.cc, dt = a.dt;
   double srg = 0.0;
   cplx<double> w[1][E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const int k = t + TPL * j;
     const double rv[2] = {v[0][j].x, v[0][j].y};
     const double cv[2] = {v[1][j].x, v[1][j].y};
     double out[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      const double q = a.phi[rbase + 2 * k + e], c = cv[e];
+      const double q = ph[j][e], c = cv[e];
       const double h = q * q * q * (10.0 - 15.0 * q + 6.0 * q * q);
       const double g = q * q * (1.0 - q) * (1.0 - q);
       const double fpp = 2.0 * A * (1.0 - h) +
@@ -298,7 +322,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     }
     w[0][j] = mk(out[0], out[1]);
   }
-  fft_regs<double, -1, L, 1, N, WAVE>(w, t, row, tw);
+  fft_regs<double, -1, L, 1, N, WAVE, E>(w, t, row, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) row[pad<double>(t + TPL * j)] = w[0][j];
   lds_sync<WAVE>();
@@ -317,10 +341,11 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
 #pragma unroll
   for (int j = 0; j < E; ++j) row[pad<double>(t + TPL * j)] = w[0][j];
   __syncthreads();
-  for (int f = threadIdx.x; f < B * L; f += NT) {
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int f = threadIdx.x + NT * r;
     const int bb = f % B, kx = f / B;
-    const size_t o = (size_t)kx * N + j0 + bb;
-    a.G[o] = a.G[o] + lds[bb * LP + pad<double>(kx)];
+    a.G[(size_t)kx * N + j0 + bb] = gp[r] + lds[bb * LP + pad<double>(kx)];
   }
   double rr[1] = {srg};
   const int ops[1] = {RED_SUM};

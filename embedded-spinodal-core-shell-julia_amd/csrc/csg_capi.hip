@@ -54,6 +54,7 @@ struct csg_ctx {
   long long nonfinite = -1;
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
+  DevParams gp{};         // parameters captured in gexec
   size_t bytes = 0;
   // adjoint: checkpoints of the forward run, segment states, adjoint field
   std::vector<void*> ckpt;
@@ -63,6 +64,12 @@ struct csg_ctx {
   double* adj_acc = nullptr;
   int64_t adj_steps = -1, adj_every = 0;
   double adj_dt = 0.0;
+  // captured full-segment graphs of the adjoint: forward steps, recompute,
+  // backward sweep (valid for adj_gK steps of adj_gdt with the current kernels)
+  hipGraphExec_t adj_fwd = nullptr, adj_rec = nullptr, adj_bwd = nullptr;
+  int64_t adj_gK = 0;
+  double adj_gdt = 0.0;
+  DevParams adj_gp{};     // parameters captured in the adjoint graphs
 };
 
 namespace {
@@ -125,13 +132,15 @@ void enqueue_p123(csg_ctx* c, const void* S) {
   c->ops.p3(a, c->tw, c->stream);
 }
 
-// P4 on state `S`; with `hist` non-NULL its workgroup 0 also writes the
-// history row of the step from the P1..P3 partials.
-void enqueue_p4(csg_ctx* c, void* S, double dt, double* hist) {
+// P4 on state `S` (read from `Sin` instead when given); with `hist` non-NULL
+// its workgroup 0 also writes the history row of the step from the P1..P3
+// partials.
+void enqueue_p4(csg_ctx* c, void* S, double dt, double* hist, const void* Sin = nullptr) {
   PassArgs a = base_args(c);
   a.in0 = c->A;
   a.in1 = c->Bb;
   a.out0 = S;
+  a.sin = Sin;
   a.dt = dt;
   a.fin = final_args(c, hist, dt);
   c->ops.p4(a, c->tw, c->stream);
@@ -188,8 +197,10 @@ int reset_ctr(csg_ctx* c) {
   return CSG_OK;
 }
 
+bool same_params(const DevParams& x, const DevParams& y) { return std::memcmp(&x, &y, sizeof(DevParams)) == 0; }
+
 int build_graph(csg_ctx* c, double dt) {
-  if (c->gexec && c->gdt == dt) return CSG_OK;
+  if (c->gexec && c->gdt == dt && same_params(c->gp, c->dp)) return CSG_OK;
   if (c->gexec) {
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
@@ -205,6 +216,7 @@ int build_graph(csg_ctx* c, double dt) {
     return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   c->gdt = dt;
+  c->gp = c->dp;
   return CSG_OK;
 }
 
@@ -212,6 +224,8 @@ int check_ctx(const csg_ctx* c) {
   if (!c) return fail(CSG_ERR_ARG, "null context");
   return CSG_OK;
 }
+
+void drop_adjoint_graphs(csg_ctx* c);
 
 }  // namespace
 
@@ -301,6 +315,7 @@ int csg_destroy(csg_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  drop_adjoint_graphs(c);
   void* bufs[] = {c->S, c->S2, c->A, c->Bb, c->phi, c->real, c->part, c->ctr, c->hist,
                   c->scratch_row, c->rate, c->tw64, c->G, c->Xr, c->adj_acc};
   for (void* b : bufs)
@@ -316,6 +331,20 @@ int csg_destroy(csg_ctx* c) {
 }
 
 int64_t csg_device_bytes(const csg_ctx* c) { return c ? static_cast<int64_t>(c->bytes) : 0; }
+
+int csg_set_params(csg_ctx* c, const csg_params* p) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!p) return fail(CSG_ERR_ARG, "params is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->p = *p;
+  c->dp = DevParams{p->A, p->B, p->chi, p->P, p->ca, p->cb, p->cc, p->kappa, p->mobility, p->Q,
+                    c->spacing, (1.0 / c->spacing) / c->n};
+  // captured graphs hold parameters in their kernel arguments: each records
+  // the parameters it was captured with and is rebuilt when used with others
+  c->adj_steps = -1;   // a checkpointed forward run belongs to the old parameters
+  return CSG_OK;
+}
 
 int csg_set_fields(csg_ctx* c, const double* c0, const double* phi) {
   if (int rc = check_ctx(c)) return rc;
@@ -650,9 +679,71 @@ int ensure_bufs(csg_ctx* c, std::vector<void*>& v, size_t count, size_t bytes) {
   return CSG_OK;
 }
 
-void enqueue_plain_step(csg_ctx* c, void* S, double dt) {
-  enqueue_p123(c, S);
-  enqueue_p4(c, S, dt, nullptr);
+// One step without history: Sout = step(Sin) (in place when equal).
+void enqueue_plain_step(csg_ctx* c, const void* Sin, void* Sout, double dt) {
+  enqueue_p123(c, Sin);
+  enqueue_p4(c, Sout, dt, nullptr, Sin == Sout ? nullptr : Sin);
+}
+
+void drop_adjoint_graphs(csg_ctx* c) {
+  for (hipGraphExec_t* g : {&c->adj_fwd, &c->adj_rec, &c->adj_bwd}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
+  }
+  c->adj_gK = 0;
+}
+
+// Capture what `body` enqueues on the context's stream into an executable graph.
+template <class F>
+int capture_graph(csg_ctx* c, hipGraphExec_t* out, F&& body) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  body();
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return CSG_OK;
+}
+
+AdjArgs adjoint_args(const csg_ctx* c) {
+  AdjArgs a{};
+  a.G = static_cast<cplx<double>*>(c->G);
+  a.X0 = static_cast<cplx<double>*>(c->A);
+  a.X1 = static_cast<cplx<double>*>(c->Bb);
+  a.Xr = static_cast<cplx<double>*>(c->Xr);
+  a.phi = static_cast<const double*>(c->phi);
+  a.dt = c->adj_dt;
+  a.p = c->dp;
+  return a;
+}
+
+// Backward sweep over steps m-1 .. 0 of a segment whose states are seg[0..m].
+void enqueue_backward_segment(csg_ctx* c, int64_t m) {
+  AdjArgs a = adjoint_args(c);
+  const int na = c->ops.n_adj_col;
+  for (int64_t i = m - 1; i >= 0; --i) {
+    a.S0 = static_cast<const cplx<double>*>(c->seg[i]);
+    a.S1 = static_cast<const cplx<double>*>(c->seg[i + 1]);
+    a.acc = c->adj_acc;
+    c->ops.adj1(a, c->tw, c->stream);
+    PassArgs pa = base_args(c);
+    pa.in0 = c->A; pa.in1 = c->Bb; pa.out0 = c->A; pa.out1 = c->Bb; pa.part = part3(c);
+    c->ops.p3(pa, c->tw, c->stream);
+    a.acc = c->adj_acc + na;
+    c->ops.adj3(a, c->tw, c->stream);
+    a.acc = c->adj_acc + 2 * na;
+    c->ops.adj4(a, c->tw, c->stream);
+  }
+}
+
+// Recompute seg[1..m] from seg[0].
+void enqueue_recompute_segment(csg_ctx* c, int64_t m) {
+  for (int64_t i = 0; i < m; ++i) enqueue_plain_step(c, c->seg[i], c->seg[i + 1], c->adj_dt);
 }
 
 }  // namespace
@@ -666,19 +757,35 @@ int csg_adjoint_forward(csg_ctx* c, double dt, int64_t steps, int64_t ckpt_every
   if (steps < 1 || ckpt_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/ckpt_every/dt");
   HIPCHK(hipSetDevice(c->device));
   const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
-  const size_t nck = static_cast<size_t>(steps / ckpt_every) + 1;
+  const int64_t K = ckpt_every, nfull = steps / K;
+  const size_t nck = static_cast<size_t>(nfull) + 1;
   if (int rc = ensure_bufs(c, c->ckpt, nck, bytes)) return rc;
-  HIPCHK(hipMemcpyAsync(c->ckpt[0], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
-  for (int64_t s = 0; s < steps; ++s) {
-    enqueue_plain_step(c, c->S, dt);
-    if ((s + 1) % ckpt_every == 0)
-      HIPCHK(hipMemcpyAsync(c->ckpt[(s + 1) / ckpt_every], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  c->adj_dt = dt;
+  if (c->adj_gK != K || c->adj_gdt != dt || !same_params(c->adj_gp, c->dp)) drop_adjoint_graphs(c);
+  // full segments replay one captured graph of K steps when there are several
+  if (nfull >= 2 && !c->adj_fwd) {
+    if (int rc = capture_graph(c, &c->adj_fwd, [&]() {
+          for (int64_t s = 0; s < K; ++s) enqueue_plain_step(c, c->S, c->S, dt);
+        }))
+      return rc;
+    c->adj_gK = K;
+    c->adj_gdt = dt;
+    c->adj_gp = c->dp;
   }
+  HIPCHK(hipMemcpyAsync(c->ckpt[0], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  for (int64_t k = 0; k < nfull; ++k) {
+    if (c->adj_fwd) {
+      HIPCHK(hipGraphLaunch(c->adj_fwd, c->stream));
+    } else {
+      for (int64_t s = 0; s < K; ++s) enqueue_plain_step(c, c->S, c->S, dt);
+    }
+    HIPCHK(hipMemcpyAsync(c->ckpt[k + 1], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  for (int64_t s = nfull * K; s < steps; ++s) enqueue_plain_step(c, c->S, c->S, dt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->adj_steps = steps;
   c->adj_every = ckpt_every;
-  c->adj_dt = dt;
   return CSG_OK;
 }
 
@@ -700,35 +807,32 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
   // gbar_T into the state's packed line layout
   if (int rc = upload_real(c, dL_dcT)) return rc;
   if (int rc = set_state_from_real(c, c->G)) return rc;
-  AdjArgs a{};
-  a.G = static_cast<cplx<double>*>(c->G);
-  a.X0 = static_cast<cplx<double>*>(c->A);
-  a.X1 = static_cast<cplx<double>*>(c->Bb);
-  a.Xr = static_cast<cplx<double>*>(c->Xr);
-  a.phi = static_cast<const double*>(c->phi);
-  a.dt = dt;
-  a.p = c->dp;
-  const int64_t nseg = (T + K - 1) / K;
+  const int64_t nseg = (T + K - 1) / K, nfull = T / K;
+  (void)dt;
+  // full segments replay captured recompute and backward graphs when there are several
+  if (c->adj_gK && (c->adj_gK != K || c->adj_gdt != c->adj_dt || !same_params(c->adj_gp, c->dp)))
+    drop_adjoint_graphs(c);
+  if (nfull >= 2 && (!c->adj_rec || !c->adj_bwd)) {
+    if (!c->adj_rec) {
+      if (int rc = capture_graph(c, &c->adj_rec, [&]() { enqueue_recompute_segment(c, K); })) return rc;
+    }
+    if (!c->adj_bwd) {
+      if (int rc = capture_graph(c, &c->adj_bwd, [&]() { enqueue_backward_segment(c, K); })) return rc;
+    }
+    c->adj_gK = K;
+    c->adj_gdt = c->adj_dt;
+    c->adj_gp = c->dp;
+  }
   for (int64_t sg = nseg - 1; sg >= 0; --sg) {
     const int64_t s0 = sg * K, s1 = std::min<int64_t>(s0 + K, T), m = s1 - s0;
-    // recompute the segment's states from its checkpoint
+    // recompute the segment's states from its checkpoint, then sweep back
     HIPCHK(hipMemcpyAsync(c->seg[0], c->ckpt[sg], bytes, hipMemcpyDeviceToDevice, c->stream));
-    for (int64_t i = 0; i < m; ++i) {
-      HIPCHK(hipMemcpyAsync(c->seg[i + 1], c->seg[i], bytes, hipMemcpyDeviceToDevice, c->stream));
-      enqueue_plain_step(c, c->seg[i + 1], dt);
-    }
-    for (int64_t i = m - 1; i >= 0; --i) {
-      a.S0 = static_cast<const cplx<double>*>(c->seg[i]);
-      a.S1 = static_cast<const cplx<double>*>(c->seg[i + 1]);
-      a.acc = c->adj_acc;
-      c->ops.adj1(a, c->tw, c->stream);
-      PassArgs pa = base_args(c);
-      pa.in0 = c->A; pa.in1 = c->Bb; pa.out0 = c->A; pa.out1 = c->Bb; pa.part = part3(c);
-      c->ops.p3(pa, c->tw, c->stream);
-      a.acc = c->adj_acc + na;
-      c->ops.adj3(a, c->tw, c->stream);
-      a.acc = c->adj_acc + 2 * na;
-      c->ops.adj4(a, c->tw, c->stream);
+    if (m == K && c->adj_rec && c->adj_bwd) {
+      HIPCHK(hipGraphLaunch(c->adj_rec, c->stream));
+      HIPCHK(hipGraphLaunch(c->adj_bwd, c->stream));
+    } else {
+      enqueue_recompute_segment(c, m);
+      enqueue_backward_segment(c, m);
     }
   }
   HIPCHK(hipGetLastError());
@@ -811,11 +915,12 @@ int csg_set_schedule(csg_ctx* c, const int32_t* sel) {
   for (int q = 0; q < 4; ++q) same = same && s[q] == c->ops.sel[q];
   if (same) return CSG_OK;
   if (!select_schedule(c->ops, s)) return fail(CSG_ERR_ARG, "schedule index out of range (csg_schedule_info)");
-  if (c->gexec) {   // the captured loop holds the previous kernels
-    HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipSetDevice(c->device));
+  if (c->gexec) {   // the captured loops hold the previous kernels
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
   }
+  drop_adjoint_graphs(c);
   return CSG_OK;
 }
 
@@ -860,6 +965,7 @@ int csg_autotune(csg_ctx* c, double dt, int32_t rounds, int32_t* sel_out, double
     (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
   }
+  drop_adjoint_graphs(c);
   HIPCHK(hipMemcpyAsync(c->S, c->S2, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (rc) return rc;

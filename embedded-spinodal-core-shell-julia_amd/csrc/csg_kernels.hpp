@@ -89,6 +89,7 @@ struct PassArgs {
   FinalArgs fin;     // P4 only
   InstanceStrides is;
   int ninst;         // instances (gridDim.y); 0 or 1: a single field
+  const void* sin;   // P4: state read from here and the sum written to out0 (NULL: in place)
 };
 
 __device__ __forceinline__ const void* offset_bytes(const void* p, long long q, long long s) {
@@ -124,6 +125,7 @@ __device__ __forceinline__ PassArgs instance_view(const PassArgs& a0) {
   a.out0 = offset_bytes(a0.out0, q, a0.is.out0);
   a.out1 = offset_bytes(a0.out1, q, a0.is.out1);
   a.phi = offset_bytes(a0.phi, q, a0.is.phi);
+  a.sin = offset_bytes(a0.sin, q, a0.is.out0);
   if (a0.part) a.part = a0.part + q * a0.is.part;
   if (a0.is.params) a.p = a0.is.params[q];
   a.fin = final_view(a0.fin, q);
@@ -543,7 +545,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       v[1][j] = X1[base + t + TPL * j];
     }
     if constexpr (MODE == COL_P4 && Cfg::PF) {
-      const cplx<TS>* S = static_cast<const cplx<TS>*>(a.out0);
+      const cplx<TS>* S = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
 #pragma unroll
       for (int j = 0; j < E; ++j) sp[j] = S[base + t + TPL * j];
     }
@@ -675,8 +677,9 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       // S += IFFT_j(inc)
       cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
       if constexpr (!Cfg::PF) {
+        const cplx<TS>* Sin = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
 #pragma unroll
-        for (int j = 0; j < E; ++j) sp[j] = S[base + t + TPL * j];
+        for (int j = 0; j < E; ++j) sp[j] = Sin[base + t + TPL * j];
       }
 #pragma unroll
       for (int j = 0; j < E; ++j) {

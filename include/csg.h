@@ -144,6 +144,11 @@ int csg_adjoint_backward(csg_ctx* ctx, const double* dL_dcT, double* dL_dc0, dou
 /* Bytes of device memory held by the context. */
 int64_t csg_device_bytes(const csg_ctx* ctx);
 
+/* Replace the context's Parameters (core_shell.jl:4-15) without reallocating
+ * anything; the resident field is kept.  A checkpointed forward run
+ * (csg_adjoint_forward) is discarded. */
+int csg_set_params(csg_ctx* ctx, const csg_params* p);
+
 /* ---- kernel schedules ------------------------------------------------------
  * No counterpart in the reference (FFTW plans its transforms itself).  Each
  * of the four pass kernels of a step (pass 0..3 = P1 row, P2 column, P3 row,

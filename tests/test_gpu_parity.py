@@ -244,3 +244,26 @@ def test_results_directory(pkg, tmp_path):
     h = np.loadtxt(d / "history.csv", delimiter=",")
     assert np.array_equal(comp, c) and np.array_equal(part, phi) and np.array_equal(h, hist)
     assert h.shape == (6, 6)
+
+
+def test_set_params_matches_fresh_context(pkg):
+    """csg_set_params: the captured loop picks up the new parameters (it was
+    built with the old ones) and the run equals a fresh context's bit for bit."""
+    say("set_params vs fresh context")
+    n = 64
+    c, phi = _fields(pkg, n)
+    p2 = pkg.Parameters(B=5.0, chi=0.7, kappa=1.5)
+    s = pkg.Session(n)
+    s.set_fields(c, phi)
+    s.run(0.01, 100)                     # captures the loop with the defaults
+    s.set_params(p2)
+    s.set_fields(c, phi)
+    h = s.run(0.01, 100)
+    f = pkg.Session(n, 1.0, p2)
+    f.set_fields(c, phi)
+    hf = f.run(0.01, 100)
+    assert np.array_equal(s.field(), f.field()) and np.array_equal(h, hf)
+    cr, _ = o.run_simulation_fast(c, phi, steps=100, p=o.Parameters(B=5.0, chi=0.7, kappa=1.5))
+    assert np.max(np.abs(s.field() - cr)) < 1e-12
+    s.close()
+    f.close()
