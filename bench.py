@@ -6,6 +6,8 @@ default Parameters, fields resident in HBM before the timed region.  The FP64
 form and the mixed form (FP32 FFTs/pointwise, FP64 state and reductions) run
 side by side on the same inputs; the mixed form is reported as `value` when
 its max|dc| against the FP64 GPU run stays <= 1e-5, else the FP64 form.
+Before each timed run the library's runtime tuner (csg_autotune) picks the
+kernel schedules on the resident field (--no-tune keeps the defaults).
 
 Also reported:
   roofline      dominant pass of the reported form: algorithmic bytes per
@@ -52,6 +54,8 @@ def _args():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default kernel schedules (no csg_autotune before the timed runs)")
     ap.add_argument("--large-steps", type=int, default=5000, help="steps of the 4096^2 field (0: skip)")
     ap.add_argument("--ens-count", type=int, default=256, help="ensemble members (0: skip the ensemble)")
     ap.add_argument("--ens-steps", type=int, default=2000)
@@ -97,6 +101,11 @@ def _traffic(kernel: str, workload: str):
 def run_form(pkg, form, c0, phi, a):
     s = pkg.Session(a.n, 1.0, pkg.Parameters(), form)
     s.set_fields(c0, phi)
+    tune = None
+    if not a.no_tune:
+        t0 = time.perf_counter()
+        sel, us = s.autotune(a.dt, 2)            # runtime tuner, resident field kept
+        tune = dict(schedule=list(sel), tuned_us_per_step=us, seconds=time.perf_counter() - t0)
     if a.warmup:
         s.run(a.dt, a.warmup)                 # also captures the step graph
     s.set_fields(c0, phi)
@@ -105,8 +114,9 @@ def run_form(pkg, form, c0, phi, a):
     t1 = time.perf_counter()
     c = s.field()
     ms_pass = s.time_passes(a.dt, 50)
+    sched = list(s.schedule)
     s.close()
-    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass)
+    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, tune=tune)
 
 
 def gradient_rate(pkg, a):
@@ -154,6 +164,8 @@ def large_field(pkg, a):
     for form in ("fp64", "mixed"):
         s = pkg.Session(n, 1.0, p, form)
         s.set_fields(c0, phi)
+        if not a.no_tune:
+            s.autotune(a.dt, 1)
         s.run(a.dt, 64)                            # captures the step graph
         s.set_fields(c0, phi)
         t0 = time.perf_counter()
@@ -195,6 +207,11 @@ def ensemble_rate(pkg, a):
         ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision=form)
         for i, (c0, phi) in enumerate(fields):
             ens.set_fields(i, c0, phi)
+        tuned = None
+        if not a.no_tune:
+            t0 = time.perf_counter()
+            tuned = {str(n): list(sel) for n, sel in ens.autotune(a.dt, 1).items()}   # per size class
+            tuned["seconds"] = time.perf_counter() - t0
         ens.run(a.dt, 64, histories=False)           # warm-up: graph capture, caches
         for i, (c0, phi) in enumerate(fields):
             ens.set_fields(i, c0, phi)
@@ -202,7 +219,7 @@ def ensemble_rate(pkg, a):
         hs = ens.run(a.dt, a.ens_steps)
         secs = time.perf_counter() - t0
         res[form] = dict(secs=secs, hist=hs, c=[ens.field(i) for i in range(len(ms))],
-                         bytes=ens.device_bytes())
+                         bytes=ens.device_bytes(), tuned=tuned)
         ens.close()
     iters = a.ens_steps + 1
     out = {"workload": f"ensemble of {len(ms)} particles, N in {sizes} "
@@ -213,7 +230,8 @@ def ensemble_rate(pkg, a):
         byts = sites * sum(PASS_BYTES[form]) * iters   # packed line layout: no N/2+1 column
         gbs = byts / r["secs"] / 1e9
         out[form] = {"site_updates_per_s": sites * iters / r["secs"], "ms_per_step": 1e3 * r["secs"] / iters,
-                     "algorithmic_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "device_bytes": r["bytes"]}
+                     "algorithmic_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "device_bytes": r["bytes"],
+                     "autotuned_schedules": r["tuned"]}
     dc = [float(np.max(np.abs(x - y))) for x, y in zip(res["mixed"]["c"], res["fp64"]["c"])]
     de = [float(np.max(np.abs(x[:, 2] - y[:, 2]) / np.abs(y[:, 2])))
           for x, y in zip(res["mixed"]["hist"], res["fp64"]["hist"])]
@@ -283,7 +301,8 @@ def main():
                  dominant=PASS_NAMES[dom], dominant_gbs=per[dom], dominant_ms=float(ms[dom]),
                  dominant_bytes=PASS_BYTES[f][dom] * sites,
                  energy_final=float(r["hist"][-1, 2]),
-                 mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))))
+                 mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))),
+                 schedule=r["schedule"], autotune=r["tune"])
         if ref is not None and f != "fp64":
             o["max_dc_vs_fp64"] = float(np.max(np.abs(r["c"] - ref["c"])))
             o["max_dc_vs_fp64_1000_steps"] = short_horizon_error(pkg, f, c0, phi, a)
