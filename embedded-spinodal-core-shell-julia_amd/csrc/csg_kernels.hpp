@@ -501,7 +501,10 @@ struct ColCfg {
   static constexpr int LP = padded_len<CT>(N);
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
-  static constexpr int GRID = (N / 2) / C;
+  static constexpr int GRID = (N / 2) / C;   // line workgroups
+  // P4 gets one more workgroup (index 0) that reduces the step's partials into
+  // the history row alongside the line workgroups instead of after them
+  static constexpr int LAUNCH = GRID + (MODE == COL_P4 ? 1 : 0);
   static_assert(fft_E_valid(N, E) && (N / 2) % C == 0, "column schedule");
   static_assert(WAVE || C == 1, "block-synchronised lines need one line per workgroup");
   static_assert(NT <= 1024 && (NT % 64 == 0 || (NT < 64 && (NT & (NT - 1)) == 0)), "workgroup of whole wavefronts");
@@ -518,8 +521,16 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
   const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;
+  if constexpr (MODE == COL_P4) {
+    // the partials of P1..P3 of this step are complete: history row
+    if (blockIdx.x == 0) {
+      if (a.fin.hist != nullptr) finalize_block<NT>(a.fin, scratch);
+      return;
+    }
+  }
+  const int lb = MODE == COL_P4 ? static_cast<int>(blockIdx.x) - 1 : static_cast<int>(blockIdx.x);
   const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
-  const int kx = blockIdx.x * C + l;
+  const int kx = lb * C + l;
   cplx<CT>* line = lds + l * LP;
   const size_t base = (size_t)kx * N;
   const DevParams& p = a.p;
@@ -688,8 +699,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
         s.y += static_cast<TS>(w[0][j].y);
         S[base + t + TPL * j] = s;
       }
-      // the partials of P1..P3 of this step are complete: history row
-      if (blockIdx.x == 0 && a.fin.hist != nullptr) finalize_block<NT>(a.fin, scratch);
+
     }
   }
 }

@@ -66,7 +66,7 @@ void launch_row(const PassArgs& a, const void* tw, hipStream_t s) {
 template <class CT, class TS, int N, int MODE, class SC = Sched<>>
 void launch_col(const PassArgs& a, const void* tw, hipStream_t s) {
   using Cfg = ColCfg<CT, TS, N, MODE, SC>;
-  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE, SC>), dim3(Cfg::GRID, instances(a)), dim3(Cfg::NT), 0, s, a,
+  hipLaunchKernelGGL((col_pass<CT, TS, N, MODE, SC>), dim3(Cfg::LAUNCH, instances(a)), dim3(Cfg::NT), 0, s, a,
                      static_cast<const cplx<CT>*>(tw));
 }
 
