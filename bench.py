@@ -255,6 +255,21 @@ def short_horizon_error(pkg, form, c0, phi, a, steps=1000):
     return float(np.max(np.abs(out[1] - out[0])))
 
 
+def cpu_threads_baseline(c0, phi, a, threads):
+    """The same oracle loop with its FFTs on `threads` threads (scipy pocketfft)."""
+    from oracle import core_shell_oracle as orc
+    orc.set_fft_workers(threads)
+    try:
+        t0 = time.perf_counter()
+        orc.run_simulation(np.asarray(c0), np.asarray(phi), dt=a.dt, steps=a.cpu_steps)
+        secs = time.perf_counter() - t0
+    finally:
+        orc.set_fft_workers(0)
+    return dict(value=a.n * a.n * (a.cpu_steps + 1) / secs, unit="site-updates/s", cores=threads, kind="port",
+                sample=f"oracle run_simulation with 2D FFTs on {threads} threads (scipy pocketfft), "
+                       f"{a.cpu_steps + 1} loop iterations of the {a.n}^2 workload in {secs:.1f} s")
+
+
 def cpu_baseline(c0, phi, a, c_gpu_steps=None):
     from oracle import core_shell_oracle as orc
     c0 = np.asarray(c0)
@@ -344,6 +359,8 @@ def main():
         base["max_dc_gpu_fp64_vs_oracle"] = float(np.max(np.abs(s.field() - c_cpu)))
         s.close()
         line["cpu_baseline"] = base
+        threads = min(16, len(os.sched_getaffinity(0)))
+        line["cpu_baseline_threads"] = cpu_threads_baseline(c0, phi, a, threads)
     print(json.dumps(line), flush=True)
 
 
