@@ -192,7 +192,8 @@ int ensure_hist(csg_ctx* c, size_t rows) {
 }
 
 int reset_ctr(csg_ctx* c) {
-  const long long init[2] = {0, -1};
+  // static storage: the asynchronous copy may read the source after return
+  static const long long init[2] = {0, -1};
   HIPCHK(hipMemcpyAsync(c->ctr, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   return CSG_OK;
 }
