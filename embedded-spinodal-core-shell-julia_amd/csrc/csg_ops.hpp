@@ -10,7 +10,7 @@ namespace csg {
 
 using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
 
-constexpr int MAX_CANDS = 3;
+constexpr int MAX_CANDS = 4;
 
 // Compiled schedules of one step pass.
 struct PassCands {

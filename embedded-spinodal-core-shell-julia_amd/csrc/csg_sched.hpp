@@ -5,10 +5,13 @@
 // workgroup, tile-load group size with 0 = all at once, prefetch): the
 // three fastest distinct (E, W) choices of an exhaustive sweep over valid
 // schedules, each pass timed alone on MI355X over 40 back-to-back launches
-// on a 0.3 N particle field (the times are in the comments).  The first entry
-// is the default; the runtime tuner (csg_autotune) times the candidates inside
-// the captured step loop and keeps the fastest combination.  Sizes without an
-// entry use the single rule-based schedule Sched<>.
+// on a 0.3 N particle field (the times are in the comments).  For the
+// ensemble sizes 128 .. 512 a fourth entry is the fastest schedule of the same
+// sweep run over 64 instances per launch, when it is not among the three.  The
+// first entry is the default; the runtime tuner (csg_autotune,
+// csg_batch_autotune) times the candidates inside the captured step loop and
+// keeps the fastest combination.  Sizes without an entry use the single
+// rule-based schedule Sched<>.
 #pragma once
 #include "csg_kernels.hpp"
 
@@ -22,13 +25,13 @@ struct StepCands {
   using type = SchedList<Sched<>>;
 };
 
-template <> struct StepCands<128, true, 1> { using type = SchedList<Sched<4, 4, 0, 1>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>>; };  // 4.8 / 5.0 / 5.9 us
+template <> struct StepCands<128, true, 1> { using type = SchedList<Sched<4, 4, 0, 1>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>, Sched<8, 16, 1, 0>>; };  // 4.8 / 5.0 / 5.9; ensemble (64 instances per launch): 8.8 us
 template <> struct StepCands<128, true, 2> { using type = SchedList<Sched<4, 2, 0, 1>, Sched<4, 4, 0, 1>, Sched<8, 4, 0, 1>>; };  // 4.0 / 4.4 / 5.4 us
-template <> struct StepCands<128, true, 3> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>>; };  // 3.7 / 4.0 / 4.5 us
+template <> struct StepCands<128, true, 3> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>, Sched<8, 8, 1, 0>>; };  // 3.7 / 4.0 / 4.5; ensemble (64 instances per launch): 10.9 us
 template <> struct StepCands<128, true, 4> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 4, 0, 0>, Sched<4, 8, 0, 0>>; };  // 4.0 / 4.3 / 4.9 us
-template <> struct StepCands<128, false, 1> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 0, 0>, Sched<4, 16, 1, 0>>; };  // 4.3 / 4.7 / 5.4 us
+template <> struct StepCands<128, false, 1> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 0, 0>, Sched<4, 16, 1, 0>, Sched<4, 8, 1, 0>>; };  // 4.3 / 4.7 / 5.4; ensemble (64 instances per launch): 6.9 us
 template <> struct StepCands<128, false, 2> { using type = SchedList<Sched<4, 2, 0, 1>, Sched<4, 4, 0, 1>, Sched<4, 8, 0, 1>>; };  // 3.7 / 4.1 / 4.5 us
-template <> struct StepCands<128, false, 3> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>>; };  // 3.4 / 3.6 / 4.0 us
+template <> struct StepCands<128, false, 3> { using type = SchedList<Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>, Sched<4, 16, 1, 0>, Sched<8, 16, 1, 0>>; };  // 3.4 / 3.6 / 4.0; ensemble (64 instances per launch): 6.9 us
 template <> struct StepCands<128, false, 4> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 4, 0, 0>, Sched<4, 8, 0, 0>>; };  // 3.8 / 3.9 / 4.3 us
 template <> struct StepCands<192, true, 1> { using type = SchedList<Sched<6, 4, 1, 1>, Sched<6, 8, 1, 0>, Sched<6, 16, 1, 0>>; };  // 7.0 / 7.5 / 8.9 us
 template <> struct StepCands<192, true, 2> { using type = SchedList<Sched<6, 2, 0, 1>, Sched<6, 4, 0, 1>, Sched<12, 4, 0, 1>>; };  // 7.0 / 7.9 / 8.9 us
@@ -39,28 +42,28 @@ template <> struct StepCands<192, false, 2> { using type = SchedList<Sched<6, 2,
 template <> struct StepCands<192, false, 3> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 8, 0, 0>, Sched<6, 16, 1, 0>>; };  // 5.0 / 5.3 / 5.9 us
 template <> struct StepCands<192, false, 4> { using type = SchedList<Sched<6, 2, 0, 0>, Sched<6, 4, 0, 0>, Sched<6, 8, 0, 0>>; };  // 5.7 / 6.1 / 7.0 us
 template <> struct StepCands<256, true, 1> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 4, 0, 0>, Sched<4, 8, 0, 0>>; };  // 5.3 / 5.5 / 6.4 us
-template <> struct StepCands<256, true, 2> { using type = SchedList<Sched<4, 1, 0, 1>, Sched<4, 2, 0, 1>, Sched<8, 2, 0, 1>>; };  // 4.3 / 4.6 / 5.7 us
-template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0>, Sched<4, 4, 1, 0>, Sched<4, 8, 1, 0>>; };  // 4.4 / 4.4 / 5.1 us
-template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<4, 2, 0, 1>, Sched<4, 4, 0, 0>>; };  // 4.0 / 4.4 / 5.1 us
-template <> struct StepCands<256, false, 1> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 4, 1, 0>, Sched<4, 8, 1, 0>>; };  // 4.8 / 5.0 / 5.6 us
-template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1>, Sched<4, 2, 0, 1>, Sched<4, 4, 0, 1>>; };  // 3.8 / 4.2 / 4.9 us
-template <> struct StepCands<256, false, 3> { using type = SchedList<Sched<4, 2, 1, 1>, Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>>; };  // 3.8 / 4.0 / 4.2 us
+template <> struct StepCands<256, true, 2> { using type = SchedList<Sched<4, 1, 0, 1>, Sched<4, 2, 0, 1>, Sched<8, 2, 0, 1>, Sched<8, 4, 0, 1>>; };  // 4.3 / 4.6 / 5.7; ensemble (64 instances per launch): 25.3 us
+template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0>, Sched<4, 4, 1, 0>, Sched<4, 8, 1, 0>, Sched<8, 16, 0, 0>>; };  // 4.4 / 4.4 / 5.1; ensemble (64 instances per launch): 34.2 us
+template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<4, 2, 0, 1>, Sched<4, 4, 0, 0>, Sched<8, 8, 0, 0>>; };  // 4.0 / 4.4 / 5.1; ensemble (64 instances per launch): 22.8 us
+template <> struct StepCands<256, false, 1> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 4, 1, 0>, Sched<4, 8, 1, 0>, Sched<4, 4, 0, 0>>; };  // 4.8 / 5.0 / 5.6; ensemble (64 instances per launch): 20.4 us
+template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1>, Sched<4, 2, 0, 1>, Sched<4, 4, 0, 1>, Sched<8, 4, 0, 1>>; };  // 3.8 / 4.2 / 4.9; ensemble (64 instances per launch): 17.4 us
+template <> struct StepCands<256, false, 3> { using type = SchedList<Sched<4, 2, 1, 1>, Sched<4, 4, 0, 0>, Sched<4, 8, 1, 0>, Sched<8, 8, 1, 0>>; };  // 3.8 / 4.0 / 4.2; ensemble (64 instances per launch): 20.1 us
 template <> struct StepCands<256, false, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<4, 2, 0, 0>, Sched<4, 4, 0, 0>>; };  // 3.7 / 4.0 / 4.5 us
-template <> struct StepCands<384, true, 1> { using type = SchedList<Sched<6, 2, 1, 1>, Sched<6, 4, 1, 0>, Sched<6, 8, 1, 0>>; };  // 8.0 / 8.2 / 9.7 us
-template <> struct StepCands<384, true, 2> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 1>, Sched<12, 2, 0, 1>>; };  // 8.1 / 8.9 / 10.3 us
-template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 2, 1, 1>, Sched<6, 8, 1, 0>>; };  // 6.9 / 7.8 / 8.2 us
-template <> struct StepCands<384, true, 4> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 0>, Sched<6, 4, 0, 0>>; };  // 7.2 / 7.7 / 9.8 us
+template <> struct StepCands<384, true, 1> { using type = SchedList<Sched<6, 2, 1, 1>, Sched<6, 4, 1, 0>, Sched<6, 8, 1, 0>, Sched<6, 4, 0, 0>>; };  // 8.0 / 8.2 / 9.7; ensemble (64 instances per launch): 79.0 us
+template <> struct StepCands<384, true, 2> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 1>, Sched<12, 2, 0, 1>, Sched<12, 8, 0, 1>>; };  // 8.1 / 8.9 / 10.3; ensemble (64 instances per launch): 76.6 us
+template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 2, 1, 1>, Sched<6, 8, 1, 0>, Sched<12, 4, 1, 0>>; };  // 6.9 / 7.8 / 8.2; ensemble (64 instances per launch): 107.4 us
+template <> struct StepCands<384, true, 4> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 0>, Sched<6, 4, 0, 0>, Sched<12, 4, 0, 0>>; };  // 7.2 / 7.7 / 9.8; ensemble (64 instances per launch): 62.8 us
 template <> struct StepCands<384, false, 1> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 2, 0, 0>, Sched<6, 8, 1, 0>>; };  // 7.2 / 7.3 / 8.2 us
-template <> struct StepCands<384, false, 2> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 1>, Sched<12, 2, 0, 1>>; };  // 6.7 / 7.1 / 8.7 us
-template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 2, 1, 1>, Sched<6, 8, 1, 0>>; };  // 6.0 / 6.2 / 6.5 us
-template <> struct StepCands<384, false, 4> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 0>, Sched<6, 4, 0, 0>>; };  // 6.1 / 6.6 / 7.8 us
-template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 2, 1, 0>, Sched<4, 1, 1, 0>, Sched<4, 4, 1, 0>>; };  // 6.2 / 6.5 / 6.9 us
-template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<8, 1, 0, 1>, Sched<4, 1, 0, 1>, Sched<8, 2, 0, 1>>; };  // 6.7 / 6.8 / 6.9 us
-template <> struct StepCands<512, true, 3> { using type = SchedList<Sched<4, 4, 1, 0>, Sched<8, 4, 1, 0>, Sched<4, 2, 1, 0>>; };  // 6.0 / 6.6 / 6.8 us
-template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<8, 1, 0, 0>, Sched<8, 2, 0, 1>>; };  // 6.0 / 6.1 / 6.3 us
-template <> struct StepCands<512, false, 1> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 1, 1, 0>, Sched<4, 4, 1, 0>>; };  // 5.5 / 5.6 / 5.9 us
+template <> struct StepCands<384, false, 2> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 1>, Sched<12, 2, 0, 1>, Sched<12, 8, 0, 1>>; };  // 6.7 / 7.1 / 8.7; ensemble (64 instances per launch): 44.9 us
+template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 2, 1, 1>, Sched<6, 8, 1, 0>, Sched<12, 4, 0, 0>>; };  // 6.0 / 6.2 / 6.5; ensemble (64 instances per launch): 49.7 us
+template <> struct StepCands<384, false, 4> { using type = SchedList<Sched<6, 1, 0, 1>, Sched<6, 2, 0, 0>, Sched<6, 4, 0, 0>, Sched<12, 2, 0, 0>>; };  // 6.1 / 6.6 / 7.8; ensemble (64 instances per launch): 41.3 us
+template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 2, 1, 0>, Sched<4, 1, 1, 0>, Sched<4, 4, 1, 0>, Sched<4, 4, 0, 1>>; };  // 6.2 / 6.5 / 6.9; ensemble (64 instances per launch): 131.2 us
+template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<8, 1, 0, 1>, Sched<4, 1, 0, 1>, Sched<8, 2, 0, 1>, Sched<8, 4, 0, 1>>; };  // 6.7 / 6.8 / 6.9; ensemble (64 instances per launch): 108.5 us
+template <> struct StepCands<512, true, 3> { using type = SchedList<Sched<4, 4, 1, 0>, Sched<8, 4, 1, 0>, Sched<4, 2, 1, 0>, Sched<8, 8, 0, 0>>; };  // 6.0 / 6.6 / 6.8; ensemble (64 instances per launch): 148.7 us
+template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<8, 1, 0, 0>, Sched<8, 2, 0, 1>, Sched<8, 4, 0, 1>>; };  // 6.0 / 6.1 / 6.3; ensemble (64 instances per launch): 92.3 us
+template <> struct StepCands<512, false, 1> { using type = SchedList<Sched<4, 2, 0, 0>, Sched<4, 1, 1, 0>, Sched<4, 4, 1, 0>, Sched<8, 2, 0, 1>>; };  // 5.5 / 5.6 / 5.9; ensemble (64 instances per launch): 66.4 us
 template <> struct StepCands<512, false, 2> { using type = SchedList<Sched<8, 1, 0, 1>, Sched<8, 2, 0, 1>, Sched<4, 1, 0, 1>>; };  // 5.5 / 5.8 / 5.9 us
-template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0>, Sched<4, 2, 1, 0>, Sched<4, 8, 1, 0>>; };  // 4.9 / 5.3 / 5.4 us
+template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0>, Sched<4, 2, 1, 0>, Sched<4, 8, 1, 0>, Sched<8, 16, 0, 1>>; };  // 4.9 / 5.3 / 5.4; ensemble (64 instances per launch): 81.8 us
 template <> struct StepCands<512, false, 4> { using type = SchedList<Sched<4, 1, 0, 0>, Sched<8, 1, 0, 1>, Sched<8, 2, 0, 0>>; };  // 5.3 / 5.4 / 5.6 us
 template <> struct StepCands<768, true, 1> { using type = SchedList<Sched<6, 4, 1, 0>, Sched<6, 1, 1, 0>, Sched<6, 2, 1, 0>>; };  // 11.6 / 11.9 / 12.2 us
 template <> struct StepCands<768, true, 2> { using type = SchedList<Sched<12, 2, 0, 1>, Sched<12, 1, 0, 1>, Sched<6, 1, 0, 1>>; };  // 13.1 / 13.4 / 16.4 us
