@@ -133,14 +133,16 @@ class Session:
 
     def schedule_info(self) -> list:
         """Per pass, the compiled candidates as dicts (E = elements per thread,
-        W = rows or lines per workgroup, G = tile-load group, PF = prefetch)."""
+        W = rows or lines per workgroup, G = tile-load group, PF = prefetch,
+        X = FFT exchange layout, T = 1 for per-stage twiddle tables)."""
+        keys = ("E", "W", "G", "PF", "X", "T")
         out = []
         for q in range(4):
             cnt = ctypes.c_int32()
             check(lib().csg_schedule_info(self.handle, q, ctypes.byref(cnt), None))
-            buf = (ctypes.c_int32 * (4 * cnt.value))()
+            buf = (ctypes.c_int32 * (len(keys) * cnt.value))()
             check(lib().csg_schedule_info(self.handle, q, ctypes.byref(cnt), buf))
-            out.append([dict(zip(("E", "W", "G", "PF"), buf[4 * i:4 * i + 4])) for i in range(cnt.value)])
+            out.append([dict(zip(keys, buf[len(keys) * i:len(keys) * (i + 1)])) for i in range(cnt.value)])
         return out
 
     @property

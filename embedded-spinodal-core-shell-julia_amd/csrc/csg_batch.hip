@@ -286,24 +286,17 @@ int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const dou
     if ((rc = bmalloc(b, reinterpret_cast<void**>(&k.dparams), m * sizeof(DevParams)))) return cleanup(rc);
     e = hipMemcpy(k.dparams, k.hparams.data(), m * sizeof(DevParams), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
-    // twiddles exp(-2 pi i j / N), extended precision, rounded once
-    std::vector<double> tw64(2 * nk);
-    std::vector<float> tw32(2 * nk);
-    for (int j = 0; j < nk; ++j) {
-      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * j / nk;
-      tw64[2 * j] = static_cast<double>(cosl(ang));
-      tw64[2 * j + 1] = static_cast<double>(sinl(ang));
-      tw32[2 * j] = static_cast<float>(cosl(ang));
-      tw32[2 * j + 1] = static_cast<float>(sinl(ang));
-    }
-    if ((rc = bmalloc(b, &k.tw64, tw64.size() * sizeof(double)))) return cleanup(rc);
-    e = hipMemcpy(k.tw64, tw64.data(), tw64.size() * sizeof(double), hipMemcpyHostToDevice);
+    // twiddles exp(-2 pi i j / N) and the FFT stage tables
+    const std::vector<cplx<double>> tw64 = make_twiddles<double>(nk);
+    const std::vector<cplx<float>> tw32 = precision == CSG_FP64 ? std::vector<cplx<float>>() : make_twiddles<float>(nk);
+    if ((rc = bmalloc(b, &k.tw64, tw64.size() * sizeof(cplx<double>)))) return cleanup(rc);
+    e = hipMemcpy(k.tw64, tw64.data(), tw64.size() * sizeof(cplx<double>), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
     if (precision == CSG_FP64) {
       k.tw = k.tw64;
     } else {
-      if ((rc = bmalloc(b, &k.tw, tw32.size() * sizeof(float)))) return cleanup(rc);
-      e = hipMemcpy(k.tw, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice);
+      if ((rc = bmalloc(b, &k.tw, tw32.size() * sizeof(cplx<float>)))) return cleanup(rc);
+      e = hipMemcpy(k.tw, tw32.data(), tw32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice);
       if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
     }
   }

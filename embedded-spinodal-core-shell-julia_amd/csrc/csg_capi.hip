@@ -284,27 +284,17 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->part), npart * sizeof(double)))) return cleanup(rc);
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->ctr), 2 * sizeof(long long)))) return cleanup(rc);
   if ((rc = dmalloc(c, reinterpret_cast<void**>(&c->scratch_row), 8 * sizeof(double)))) return cleanup(rc);
-  // twiddles exp(-2 pi i m / N), extended precision, rounded once
-  std::vector<double> tw64(2 * n);
-  for (int m = 0; m < n; ++m) {
-    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
-    tw64[2 * m] = static_cast<double>(cosl(ang));
-    tw64[2 * m + 1] = static_cast<double>(sinl(ang));
-  }
-  if ((rc = dmalloc(c, &c->tw64, 2 * n * sizeof(double)))) return cleanup(rc);
-  e = hipMemcpy(c->tw64, tw64.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice);
+  // twiddles exp(-2 pi i m / N) and the FFT stage tables
+  const std::vector<cplx<double>> tw64 = make_twiddles<double>(n);
+  if ((rc = dmalloc(c, &c->tw64, tw64.size() * sizeof(cplx<double>)))) return cleanup(rc);
+  e = hipMemcpy(c->tw64, tw64.data(), tw64.size() * sizeof(cplx<double>), hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
   if (precision == CSG_FP64) {
     c->tw = c->tw64;
   } else {
-    std::vector<float> tw32(2 * n);
-    for (int m = 0; m < n; ++m) {
-      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
-      tw32[2 * m] = static_cast<float>(cosl(ang));
-      tw32[2 * m + 1] = static_cast<float>(sinl(ang));
-    }
-    if ((rc = dmalloc(c, &c->tw, 2 * n * sizeof(float)))) return cleanup(rc);
-    e = hipMemcpy(c->tw, tw32.data(), 2 * n * sizeof(float), hipMemcpyHostToDevice);
+    const std::vector<cplx<float>> tw32 = make_twiddles<float>(n);
+    if ((rc = dmalloc(c, &c->tw, tw32.size() * sizeof(cplx<float>)))) return cleanup(rc);
+    e = hipMemcpy(c->tw, tw32.data(), tw32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
   }
   *out = c;
@@ -899,7 +889,7 @@ int csg_schedule_info(const csg_ctx* c, int32_t pass, int32_t* count, int32_t* s
   *count = pc.n;
   if (sched)
     for (int i = 0; i < pc.n; ++i)
-      for (int q = 0; q < 4; ++q) sched[4 * i + q] = pc.sched[i][q];
+      for (int q = 0; q < SCHED_FIELDS; ++q) sched[SCHED_FIELDS * i + q] = pc.sched[i][q];
   return CSG_OK;
 }
 

@@ -159,8 +159,11 @@ constexpr int fft_E(int len) {
   return (len % 3 == 0) ? (len < 12 ? len : 12)
          : len <= 8 ? len : ((len == 16 || len == 128 || len == 256) ? 16 : 8);
 }
+// A valid tile leaves a radix >= 2 for every stage: E >= 2 dividing LEN, and
+// for 3 * 2^m lines a multiple of 3 that is even unless LEN = 3.
 constexpr bool fft_E_valid(int len, int e) {
-  return e >= 1 && e <= len && len % e == 0 && (len % 3 != 0 || e % 3 == 0) && e <= 32;
+  return e >= 2 && e <= len && len % e == 0 && e <= 32 &&
+         (len % 3 != 0 || (e % 3 == 0 && (len == 3 || e % 2 == 0)));
 }
 // Radix of forward stage s: 3 while 3 divides what remains, then the largest
 // power of two <= 16 dividing both E and what remains.
@@ -211,6 +214,84 @@ CSG_HD int stockham_out(int i, int q, int R, int P) {
   return (i - k) * R + k + q * P;
 }
 
+// ---- stage twiddle tables ----------------------------------------------------
+// The twiddles of a plan laid out per stage so that the lanes of a stage read
+// consecutive entries (no LDS bank conflicts): for stage s >= 1 with radix R
+// and stride product P, entry (P - 1) + (r - 1) P + k holds
+// W_LEN^(r k LEN / (P R)) with the forward sign, r in [1, R), k in [0, P).
+// The offsets telescope to LEN - 1 entries; stage 0's (P = 1) are all 1 and
+// go unused.
+// The inverse plan (radices reversed) has its own table after the forward one
+// unless both plans are the same.
+constexpr bool plan_symmetric(int len, int e) {
+  for (int s = 0; s < fft_nstages(len, e); ++s)
+    if (plan_radix(len, s, false, e) != plan_radix(len, s, true, e)) return false;
+  return true;
+}
+constexpr int stab_len(int len, int e) { return (len - 1) * (plan_symmetric(len, e) ? 1 : 2); }
+constexpr int stab_inv_off(int len, int e) { return plan_symmetric(len, e) ? 0 : len - 1; }
+
+// Exponent (units of W_LEN) of entry q of the table of plan INV.
+template <int LEN, int E, bool INV, int S = 0>
+CSG_HD int stab_exp(int q) {
+  if constexpr (S >= fft_nstages(LEN, E)) {
+    return 0;
+  } else {
+    constexpr int P = plan_pstride(LEN, S, INV, E), R = plan_radix(LEN, S, INV, E);
+    if (q < P * R - 1) {
+      const int l = q - (P - 1);
+      return (1 + l / P) * (l % P) * (LEN / (P * R));
+    }
+    return stab_exp<LEN, E, INV, S + 1>(q);
+  }
+}
+
+// The same for run-time (len, e): host code fills the tables with it.
+constexpr int stab_exp_rt(int len, int e, bool inv, int q) {
+  for (int s = 0; s < fft_nstages(len, e); ++s) {
+    const int p = plan_pstride(len, s, inv, e), r = plan_radix(len, s, inv, e);
+    if (q < p * r - 1) {
+      const int l = q - (p - 1);
+      return (1 + l / p) * (l % p) * (len / (p * r));
+    }
+  }
+  return 0;
+}
+
+// Twiddle buffer of an N-point grid: W_N^m for m in [0, N), then the stage
+// tables (forward plan, and the inverse plan's when it differs) of every
+// (LEN, E) a kernel may use, LEN in {N, N/2}, E in stab_e_list.  Kernels copy
+// their tables from the buffer to LDS with contiguous loads.
+constexpr int stab_e_count = 9;
+constexpr int stab_e_at(int i) {
+  constexpr int es[stab_e_count] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
+  return es[i];
+}
+constexpr bool stab_e_ok(int len, int e) { return len >= 2 && fft_E_valid(len, e); }
+// offset (in entries) of the (len, e) tables in the buffer of an n-point grid; -1 if none
+constexpr int stab_offset(int n, int len, int e) {
+  int off = n;
+  for (int h = 0; h < 2; ++h) {
+    const int l = h == 0 ? n : n / 2;
+    for (int i = 0; i < stab_e_count; ++i) {
+      const int x = stab_e_at(i);
+      if (!stab_e_ok(l, x)) continue;
+      if (l == len && x == e) return off;
+      off += stab_len(l, x);
+    }
+  }
+  return -1;
+}
+constexpr int twiddle_len(int n) {
+  int off = n;
+  for (int h = 0; h < 2; ++h) {
+    const int l = h == 0 ? n : n / 2;
+    for (int i = 0; i < stab_e_count; ++i)
+      if (stab_e_ok(l, stab_e_at(i))) off += stab_len(l, stab_e_at(i));
+  }
+  return off;
+}
+
 // ---- device side -------------------------------------------------------------
 
 // W_NTAB^e (0 <= e < NTAB) for direction DIR from the half table.
@@ -243,6 +324,26 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __
   }
 }
 
+// Copy the stage table(s) of the (LEN, E) plans of an N-point grid's
+// twiddle buffer g (twiddle_len(N) entries) to LDS: all loads issued
+// (clamped), then the LDS stores, as in load_twiddles.
+template <int N, int LEN, int E, int NT, class T>
+__device__ __forceinline__ void load_stage_tables(cplx<T>* st, const cplx<T>* __restrict__ g) {
+  constexpr int Q = stab_len(LEN, E), K = (Q + NT - 1) / NT, OFF = stab_offset(N, LEN, E);
+  static_assert(OFF >= N, "no stage table for this plan in the twiddle buffer");
+  cplx<T> w[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int q = static_cast<int>(threadIdx.x) + NT * r;
+    w[r] = g[OFF + (q < Q ? q : Q - 1)];
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int q = static_cast<int>(threadIdx.x) + NT * r;
+    if (q < Q) st[q] = w[r];
+  }
+}
+
 // Order LDS traffic between stages: lines that fit in one wavefront only need
 // the wave's own accesses ordered; wider lines need the workgroup barrier.
 template <bool WAVE>
@@ -256,8 +357,9 @@ __device__ __forceinline__ void lds_sync() {
   }
 }
 
-// Butterflies of one stage for one array, in registers.
-template <class T, int DIR, int LEN, int TPL, int E, int R, int P, int NTAB>
+// Butterflies of one stage for one array, in registers.  ST: `tw` is the
+// stage table of this direction's plan; otherwise the half table of W_NTAB.
+template <class T, int DIR, int LEN, int TPL, int E, int R, int P, int NTAB, bool ST = false>
 CSG_HD void stage_compute(cplx<T> (&v)[E], int t, const cplx<T>* tw) {
   constexpr int NBT = E / R;  // butterflies per thread
   constexpr int TM = NTAB / LEN;
@@ -268,7 +370,16 @@ CSG_HD void stage_compute(cplx<T> (&v)[E], int t, const cplx<T>* tw) {
       const int k = i % P;
       constexpr int STEP = LEN / (P * R);
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[m + r * NBT] = cmul(v[m + r * NBT], tw_at<DIR, NTAB>(tw, r * k * STEP * TM));
+      for (int r = 1; r < R; ++r) {
+        cplx<T> w;
+        if constexpr (ST) {
+          w = tw[(P - 1) + (r - 1) * P + k];
+          if constexpr (DIR > 0) w.y = -w.y;
+        } else {
+          w = tw_at<DIR, NTAB>(tw, r * k * STEP * TM);
+        }
+        v[m + r * NBT] = cmul(v[m + r * NBT], w);
+      }
     }
     dft<R, DIR, NBT>(&v[m]);
   }
@@ -293,7 +404,41 @@ __device__ __forceinline__ void stage_exchange(cplx<T> (&v)[E], int t, cplx<T>* 
   lds_sync<WAVE>();
 }
 
-template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE, int E>
+// Exchange layouts of a transform of NA arrays.  `line` points at array 0's
+// buffer (padded_len<T>(LEN) elements); array a's is at line + a * AS.
+//   XM = 0: the arrays are exchanged one after another (AS = 0: one buffer
+//           for all); two ordering points per array and stage.
+//   XM = 1: all arrays written, one ordering point, all read, one more.
+//   XM = 2: as 1, but odd stages use a second set of buffers at line + BS, so
+//           a stage's reads and the next stage's writes touch different
+//           memory and need no ordering point between them; one closing
+//           ordering point at the end of the transform frees the buffers.
+template <class T, int LEN, int TPL, int E, int R, int P, bool WAVE, int NA, int XM, int AS, int BS, int S>
+__device__ __forceinline__ void stage_exchange_all(cplx<T> (&v)[NA][E], int t, cplx<T>* line) {
+  if constexpr (XM == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) stage_exchange<T, LEN, TPL, E, R, P, WAVE>(v[a], t, line + a * AS);
+  } else {
+    constexpr int NBT = E / R;
+    cplx<T>* buf = line + ((XM == 2 && (S & 1)) ? BS : 0);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int m = 0; m < NBT; ++m) {
+        const int i = t + TPL * m;
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[a * AS + pad<T>(stockham_out(i, q, R, P))] = v[a][m + q * NBT];
+      }
+    lds_sync<WAVE>();
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[a][j] = buf[a * AS + pad<T>(t + TPL * j)];
+    if constexpr (XM == 1) lds_sync<WAVE>();
+  }
+}
+
+template <class T, int DIR, int LEN, int NA, int S, int NTAB, bool WAVE, int E, int XM, int AS, int BS, bool ST>
 __device__ __forceinline__ void fft_stages(cplx<T> (&v)[NA][E], int t, cplx<T>* line, const cplx<T>* tw) {
   constexpr bool INV = DIR > 0;
   constexpr int NS = fft_nstages(LEN, E);
@@ -301,22 +446,29 @@ __device__ __forceinline__ void fft_stages(cplx<T> (&v)[NA][E], int t, cplx<T>* 
     constexpr int TPL = LEN / E;
     constexpr int R = plan_radix(LEN, S, INV, E), P = plan_pstride(LEN, S, INV, E);
 #pragma unroll
-    for (int a = 0; a < NA; ++a) stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB>(v[a], t, tw);
-    if constexpr (S + 1 < NS) {
-#pragma unroll
-      for (int a = 0; a < NA; ++a) stage_exchange<T, LEN, TPL, E, R, P, WAVE>(v[a], t, line);
-    }
-    fft_stages<T, DIR, LEN, NA, S + 1, NTAB, WAVE, E>(v, t, line, tw);
+    for (int a = 0; a < NA; ++a) stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB, ST>(v[a], t, tw);
+    if constexpr (S + 1 < NS) stage_exchange_all<T, LEN, TPL, E, R, P, WAVE, NA, XM, AS, BS, S>(v, t, line);
+    fft_stages<T, DIR, LEN, NA, S + 1, NTAB, WAVE, E, XM, AS, BS, ST>(v, t, line, tw);
+  } else if constexpr (XM == 2 && NS > 1) {
+    lds_sync<WAVE>();
   }
 }
 
 // Transform NA lines held in canonical registers by this thread group (TPL =
-// LEN/E threads, t = index within the group), using `line` (padded_len<T>(LEN)
-// elements of LDS private to the group) for the exchanges.
-template <class T, int DIR, int LEN, int NA, int NTAB, bool WAVE, int E = fft_E(LEN)>
+// LEN/E threads, t = index within the group), using the group's LDS line
+// buffers at `line` (layout XM, array stride AS, buffer-set stride BS; each
+// buffer padded_len<T>(LEN) elements) for the exchanges.  Twiddles: ST, the
+// stage tables of the (LEN, E) plans (load_stage_tables); otherwise the half
+// table of W_NTAB.
+template <class T, int DIR, int LEN, int NA, int NTAB, bool WAVE, int E = fft_E(LEN), int XM = 0, int AS = 0, int BS = 0,
+          bool ST = false>
 __device__ __forceinline__ void fft_regs(cplx<T> (&v)[NA][E], int t, cplx<T>* line, const cplx<T>* tw) {
   static_assert(fft_E_valid(LEN, E), "E must divide LEN (a multiple of 3 for 3 * 2^m lines)");
-  fft_stages<T, DIR, LEN, NA, 0, NTAB, WAVE, E>(v, t, line, tw);
+  static_assert(XM == 0 || NA == 1 || AS >= padded_len<T>(LEN), "merged exchanges need one buffer per array");
+  static_assert(XM != 2 || BS >= (NA - 1) * AS + padded_len<T>(LEN), "second buffer set after the first");
+  constexpr int IO = (ST && DIR > 0) ? stab_inv_off(LEN, E) : 0;   // folded: the plan search is not cheap
+  const cplx<T>* t0 = tw + IO;
+  fft_stages<T, DIR, LEN, NA, 0, NTAB, WAVE, E, XM, AS, BS, ST>(v, t, line, t0);
 }
 
 // ---- workgroup reductions (wave shuffles, one barrier) ---------------------

@@ -3,8 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "csg_ops.hpp"
 
@@ -31,6 +33,34 @@ bool ops_for(int n, Ops* o) {
 inline std::string unsupported_size_message(int n) {
   return "grid size " + std::to_string(n) +
          " not supported (N = 2^m with 16 <= N <= 4096, or N = 3*2^m with 48 <= N <= 768)";
+}
+
+// Twiddle buffer of an N-point grid (layout: stab_offset): W_N^m, m in
+// [0, N), from extended-precision sines and cosines rounded once, then each
+// stage-table entry taken from the first half of that table (negated past
+// the half circle), so both twiddle paths use the same values.
+template <class T>
+std::vector<cplx<T>> make_twiddles(int n) {
+  std::vector<cplx<T>> w(twiddle_len(n));
+  for (int m = 0; m < n; ++m) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
+    w[m] = cplx<T>{static_cast<T>(cosl(ang)), static_cast<T>(sinl(ang))};
+  }
+  const int h = n / 2;
+  for (int hh = 0; hh < 2; ++hh) {
+    const int len = hh == 0 ? n : n / 2;
+    for (int i = 0; i < stab_e_count; ++i) {
+      const int e = stab_e_at(i);
+      if (!stab_e_ok(len, e)) continue;
+      const int off = stab_offset(n, len, e), q1 = len - 1;
+      for (int q = 0; q < stab_len(len, e); ++q) {
+        const int x = (q < q1 ? stab_exp_rt(len, e, false, q) : stab_exp_rt(len, e, true, q - q1)) * (n / len);
+        const cplx<T> v = w[x < h ? x : x - h];
+        w[off + q] = x < h ? v : cplx<T>{-v.x, -v.y};
+      }
+    }
+  }
+  return w;
 }
 
 constexpr int FIN_NT = 256;           // threads of the standalone history reduction

@@ -150,6 +150,11 @@ template <int N, class CT>
 constexpr bool tw_in_lds() { return (N / 2) * 2 * sizeof(CT) <= 16384; }
 template <int N, class CT>
 constexpr int tw_lds_len() { return tw_in_lds<N, CT>() ? N / 2 : 1; }
+// Schedules may have the FFT engine read its twiddles from per-stage tables
+// in LDS (lanes read consecutive entries) where the half table is
+// LDS-resident and the tables of the (LEN, E) plans fit 32 KiB.
+template <int N, class CT, int LEN, int E>
+constexpr bool stab_fits() { return tw_in_lds<N, CT>() && stab_len(LEN, E) * 2 * (int)sizeof(CT) <= 32 * 1024; }
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
   const double t = s + v;
@@ -221,10 +226,14 @@ constexpr int row_rows() {
 //   G   row passes: transposed-tile loads issued in groups of G, each group
 //       written to LDS before the next is issued (E = all at once),
 //   PF  1: the pass's other inputs (phi in P1/P3, the state in P4) are loaded
-//       with the first group of loads; 0: where they are used.
-template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1>
+//       with the first group of loads; 0: where they are used,
+//   X   FFT exchange layout (fft_regs XM: 0 one array at a time, 1 all
+//       arrays together, 2 all together with two buffer sets); -1 = 0,
+//   T   FFT twiddles: 1 per-stage tables copied to LDS (lanes read
+//       consecutive entries), 0 the half table; -1 = 0.
+template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1, int X_ = -1, int T_ = -1>
 struct Sched {
-  static constexpr int E = E_, W = W_, G = G_, PF = PF_;
+  static constexpr int E = E_, W = W_, G = G_, PF = PF_, X = X_, T = T_;
 };
 
 template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
@@ -237,7 +246,12 @@ struct RowCfg {
   static constexpr int B = SC::W ? SC::W : row_rows<N, CT, NARR, E>();
   static constexpr int G = SC::G ? SC::G : E;
   static constexpr int PF = SC::PF;
+  static constexpr int XM = SC::X >= 0 ? SC::X : 0;
   static constexpr int LP = padded_len<CT>(L);
+  static constexpr int AS = B * LP, BS = NARR * B * LP;   // exchange strides
+  static constexpr bool ST = SC::T == 1;
+  static_assert(!ST || stab_fits<N, CT, L, E>(), "stage tables do not fit this schedule");
+  static constexpr int LDS = (XM == 2 ? 2 : 1) * NARR * B * LP;
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int TILES = N / B;
@@ -254,10 +268,14 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr int NARR = Cfg::NARR;
   constexpr bool WAVE = Cfg::WAVE;
-  __shared__ cplx<CT> lds[NARR * B * LP];
+  constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
+  constexpr bool ST = Cfg::ST;
+  __shared__ cplx<CT> lds[Cfg::LDS];
   __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
+  __shared__ cplx<CT> st_s[ST ? stab_len(L, E) : 1];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
-  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;
+  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;   // real-FFT pre/post twiddles
+  const cplx<CT>* ftw = ST ? st_s : tw;                     // FFT engine twiddles
   // split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2 TILES) take Gy/Jy
   const int tile = xcd_tile(blockIdx.x, Cfg::GRID);
   const int arr0 = (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0;
@@ -291,6 +309,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
     }
     if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+    if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
     __syncthreads();
   } else {
     // transposed tile load from line layout: lane-consecutive rows (coalesced),
@@ -314,6 +333,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
         }
         if constexpr (PF) load_phi();
         if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+        if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
       }
 #pragma unroll
       for (int q = 0; q < NARR; ++q)
@@ -343,7 +363,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     lds_sync<WAVE>();
-    fft_regs<CT, +1, L, NARR, N, WAVE, E>(v, t, row, tw);
+    fft_regs<CT, +1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
   }
 
   if constexpr (MODE == ROW_INV) {
@@ -416,7 +436,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   }
 
   // R2C along i: forward FFT, then X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k})
-  fft_regs<CT, -1, L, NARR, N, WAVE, E>(v, t, row, tw);
+  fft_regs<CT, -1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
 #pragma unroll
   for (int q = 0; q < NARR; ++q) {
     cplx<CT>* tr = lds + (q * B + b) * LP;
@@ -498,7 +518,13 @@ struct ColCfg {
   static constexpr int TPL = N / E;
   static constexpr int C = SC::W ? SC::W : col_lines<N, CT, E>();
   static constexpr int PF = SC::PF;
+  static constexpr int XM = SC::X >= 0 ? SC::X : 0;
   static constexpr int LP = padded_len<CT>(N);
+  // exchange buffers: arrays at stride AS, the second set (XM = 2) at BS
+  static constexpr int AS = XM == 0 ? 0 : C * LP, BS = 2 * C * LP;
+  static constexpr int LDS = (XM == 0 ? 1 : XM == 1 ? 2 : 4) * C * LP;
+  static constexpr bool ST = SC::T == 1;
+  static_assert(!ST || stab_fits<N, CT, N, E>(), "stage tables do not fit this schedule");
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = (N / 2) / C;   // line workgroups
@@ -517,10 +543,13 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   const PassArgs a = instance_view(a0);
   constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
-  __shared__ cplx<CT> lds[C * LP];
-  __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
+  constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
+  constexpr bool ST = Cfg::ST;
+  __shared__ cplx<CT> lds[Cfg::LDS];
+  // the stage tables, or the half table
+  __shared__ cplx<CT> tw_s[ST ? stab_len(N, E) : tw_lds_len<N, CT>()];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
-  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;
+  const cplx<CT>* tw = (ST || tw_in_lds<N, CT>()) ? tw_s : tw_g;
   if constexpr (MODE == COL_P4) {
     // the partials of P1..P3 of this step are complete: history row
     if (blockIdx.x == 0) {
@@ -562,9 +591,10 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     }
   }
   // twiddles after the line loads: a single wait covers both
-  if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+  if constexpr (ST) load_stage_tables<N, N, E, NT>(tw_s, tw_g);
+  else if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<CT, -1, N, 2, N, WAVE, E>(v, t, line, tw);
+  fft_regs<CT, -1, N, 2, N, WAVE, E, XM, AS, BS, ST>(v, t, line, tw);
 
   // packed line 0 (k_x = 0 in re, k_x = N/2 in im) needs the value at -ky of
   // one array: stage it through the line buffer
@@ -616,7 +646,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     if constexpr (MODE == COL_P2) {
-      fft_regs<CT, +1, N, 2, N, WAVE, E>(v, t, line, tw);
+      fft_regs<CT, +1, N, 2, N, WAVE, E, XM, AS, BS, ST>(v, t, line, tw);
       cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
       cplx<CT>* O1 = static_cast<cplx<CT>*>(a.out1);
 #pragma unroll
@@ -632,7 +662,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       cplx<CT> w[1][E];
 #pragma unroll
       for (int j = 0; j < E; ++j) w[0][j] = v[0][j];
-      fft_regs<CT, +1, N, 1, N, WAVE, E>(w, t, line, tw);
+      fft_regs<CT, +1, N, 1, N, WAVE, E, XM, AS, BS, ST>(w, t, line, tw);
       cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
 #pragma unroll
       for (int j = 0; j < E; ++j) O0[base + t + TPL * j] = w[0][j];
@@ -684,7 +714,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
     if constexpr (MODE == COL_P4) {
-      fft_regs<CT, +1, N, 1, N, WAVE, E>(w, t, line, tw);
+      fft_regs<CT, +1, N, 1, N, WAVE, E, XM, AS, BS, ST>(w, t, line, tw);
       // S += IFFT_j(inc)
       cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
       if constexpr (!Cfg::PF) {

@@ -11,12 +11,13 @@ namespace csg {
 using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
 
 constexpr int MAX_CANDS = 4;
+constexpr int SCHED_FIELDS = 6;   // E, W, G, PF, X, T
 
 // Compiled schedules of one step pass.
 struct PassCands {
   LaunchFn fn[MAX_CANDS] = {};
   int grid[MAX_CANDS] = {};
-  int sched[MAX_CANDS][4] = {};   // resolved E, W, G, PF
+  int sched[MAX_CANDS][SCHED_FIELDS] = {};   // resolved E, W, G, PF, X, T
   int n = 0;
 };
 
@@ -95,15 +96,15 @@ void add_cand(PassCands& pc) {
     using Cfg = RowCfg<T, TIN, T, N, MODE, SC>;
     pc.fn[i] = launch_row<T, TIN, T, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[4] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF};
-    for (int q = 0; q < 4; ++q) pc.sched[i][q] = r[q];
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0};
+    for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   } else {
     constexpr int MODE = PASS == 2 ? COL_P2 : COL_P4;
     using Cfg = ColCfg<T, double, N, MODE, SC>;
     pc.fn[i] = launch_col<T, double, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[4] = {Cfg::E, Cfg::C, 0, Cfg::PF};
-    for (int q = 0; q < 4; ++q) pc.sched[i][q] = r[q];
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::C, 0, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0};
+    for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   }
 }
 

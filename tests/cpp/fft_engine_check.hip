@@ -1,7 +1,10 @@
 // Host-only check of the register Stockham engine (csg_fft.hpp): drives the
 // engine's stage_compute / plan / padding / Stockham output indexing over
 // emulated threads of one line and compares forward and inverse transforms
-// with a naive DFT for every line length the kernels use (2^m and 3*2^m).  No GPU needed.
+// with a naive DFT for every line length the kernels use (2^m and 3*2^m).
+// Each transform runs twice, with twiddles from the half table and from the
+// per-stage tables (entries placed by stab_exp, values from the half table):
+// both must match the DFT and each other bit for bit.  No GPU needed.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -11,7 +14,7 @@
 
 using namespace csg;
 
-template <int DIR, int LEN, int S, class T, int NTAB, int E>
+template <int DIR, int LEN, int S, class T, int NTAB, int E, bool ST>
 void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::vector<cplx<T>>& line) {
   constexpr bool INV = DIR > 0;
   constexpr int NS = fft_nstages(LEN, E);
@@ -22,7 +25,7 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
     for (int t = 0; t < TPL; ++t) {
       cplx<T> r[E];
       for (int j = 0; j < E; ++j) r[j] = v[t][j];
-      stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB>(r, t, tw);
+      stage_compute<T, DIR, LEN, TPL, E, R, P, NTAB, ST>(r, t, tw);
       for (int j = 0; j < E; ++j) v[t][j] = r[j];
     }
     if constexpr (S + 1 < NS) {
@@ -33,7 +36,7 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
       for (int t = 0; t < TPL; ++t)
         for (int j = 0; j < E; ++j) v[t][j] = line[pad<T>(t + TPL * j)];
     }
-    run_stages<DIR, LEN, S + 1, T, NTAB, E>(v, tw, line);
+    run_stages<DIR, LEN, S + 1, T, NTAB, E, ST>(v, tw, line);
   }
 }
 
@@ -57,17 +60,40 @@ int check(double tol) {
     double b = (s >> 8) / 16777216.0 - 0.5;
     z = {a, b};
   }
+  // stage tables of the forward and inverse plans
+  std::vector<cplx<T>> st(stab_len(LEN, E));
+  for (int q = 0; q < (int)st.size(); ++q) {
+    const int e = q < LEN - 1 ? stab_exp<LEN, E, false>(q) : stab_exp<LEN, E, true>(q - (LEN - 1));
+    if (e < 0 || e >= LEN) { printf("LEN=%d E=%d entry %d exponent %d out of range\n", LEN, E, q, e); return 1; }
+    // the run-time form the host uses to fill the twiddle buffer
+    const int er = q < LEN - 1 ? stab_exp_rt(LEN, E, false, q) : stab_exp_rt(LEN, E, true, q - (LEN - 1));
+    if (er != e) { printf("LEN=%d E=%d entry %d: stab_exp_rt %d != stab_exp %d\n", LEN, E, q, er, e); return 1; }
+    st[q] = tw_at<-1, NTAB>(tw.data(), e * (NTAB / LEN));
+  }
   int bad = 0;
   for (int dir : {-1, +1}) {
-    std::vector<std::vector<cplx<T>>> v(TPL, std::vector<cplx<T>>(E));
-    for (int t = 0; t < TPL; ++t)
-      for (int j = 0; j < E; ++j) {
-        const auto z = x[t + TPL * j];
-        v[t][j] = cplx<T>{static_cast<T>(z.real()), static_cast<T>(z.imag())};
-      }
+    auto load = [&]() {
+      std::vector<std::vector<cplx<T>>> v(TPL, std::vector<cplx<T>>(E));
+      for (int t = 0; t < TPL; ++t)
+        for (int j = 0; j < E; ++j) {
+          const auto z = x[t + TPL * j];
+          v[t][j] = cplx<T>{static_cast<T>(z.real()), static_cast<T>(z.imag())};
+        }
+      return v;
+    };
+    std::vector<std::vector<cplx<T>>> v = load(), vs = load();
     std::vector<cplx<T>> line(padded_len<T>(LEN));
-    if (dir < 0) run_stages<-1, LEN, 0, T, NTAB, E>(v, tw.data(), line);
-    else run_stages<+1, LEN, 0, T, NTAB, E>(v, tw.data(), line);
+    if (dir < 0) {
+      run_stages<-1, LEN, 0, T, NTAB, E, false>(v, tw.data(), line);
+      run_stages<-1, LEN, 0, T, NTAB, E, true>(vs, st.data(), line);
+    } else {
+      run_stages<+1, LEN, 0, T, NTAB, E, false>(v, tw.data(), line);
+      run_stages<+1, LEN, 0, T, NTAB, E, true>(vs, st.data() + stab_inv_off(LEN, E), line);
+    }
+    int diff = 0;
+    for (int t = 0; t < TPL; ++t)
+      for (int j = 0; j < E; ++j) diff += (v[t][j].x != vs[t][j].x) || (v[t][j].y != vs[t][j].y);
+    if (diff) { printf("LEN=%d E=%d dir=%+d: stage-table transform differs in %d values\n", LEN, E, dir, diff); ++bad; }
     double err = 0, mag = 0;
     for (int k = 0; k < LEN; ++k) {
       std::complex<long double> acc = 0;
@@ -87,8 +113,31 @@ int check(double tol) {
   return bad;
 }
 
+// The stage tables of every (LEN, E) of an n-point grid tile [n, twiddle_len(n))
+// of the twiddle buffer without overlap.
+int check_layout(int n) {
+  std::vector<int> owner(twiddle_len(n), -1);
+  for (int h = 0; h < 2; ++h) {
+    const int len = h == 0 ? n : n / 2;
+    for (int i = 0; i < stab_e_count; ++i) {
+      const int e = stab_e_at(i);
+      if (!stab_e_ok(len, e)) continue;
+      const int off = stab_offset(n, len, e);
+      if (off < n || off + stab_len(len, e) > twiddle_len(n)) { printf("n=%d len=%d e=%d: tables outside the buffer\n", n, len, e); return 1; }
+      for (int q = 0; q < stab_len(len, e); ++q) {
+        if (owner[off + q] != -1) { printf("n=%d len=%d e=%d: tables overlap\n", n, len, e); return 1; }
+        owner[off + q] = len * 64 + e;
+      }
+    }
+  }
+  for (int q = n; q < twiddle_len(n); ++q)
+    if (owner[q] == -1) { printf("n=%d: gap at %d\n", n, q); return 1; }
+  return 0;
+}
+
 int main() {
   int bad = 0;
+  for (int n : {16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 48, 96, 192, 384, 768}) bad += check_layout(n);
   bad += check<8, double>(1e-14);
   bad += check<16, double>(1e-14);
   bad += check<32, double>(1e-14);

@@ -110,15 +110,17 @@ def test_writedlm_layout(pkg, tmp_path):
 
 def test_fft_engine_host_check(tmp_path):
     """The register Stockham engine's stage code (plans, radix-16/8/4/2
-    butterflies, padded exchanges, twiddles) against a long-double DFT for
-    every line length 8..4096, FP64 and FP32 -- compiled for the host."""
+    butterflies, padded exchanges, twiddles from the half table and from the
+    per-stage tables) against a long-double DFT for every line length
+    8..4096, FP64 and FP32, and the twiddle-buffer layout -- compiled for the
+    host."""
     import subprocess
     exe = tmp_path / "fft_engine_check"
     src = os.path.join(ROOT, "tests", "cpp", "fft_engine_check.hip")
     r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-std=c++17", src, "-o", str(exe)],
-                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:]
-    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout
 
 
