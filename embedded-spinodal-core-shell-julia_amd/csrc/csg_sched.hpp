@@ -10,7 +10,9 @@
 // the fastest of the same sweep run over 64 instances per launch when it is
 // not among them.  Each choice then carries the fastest (X, T) of a second
 // sweep over those, and the list is ordered by that sweep's times (in the
-// comments).  The first entry is the default; the runtime tuner
+// comments).  For the ensemble sizes, the fastest (E, W, G, PF, X, T) of the
+// second sweep run over 64 instances per launch is appended when it is not
+// already listed.  The first entry is the default; the runtime tuner
 // (csg_autotune, csg_batch_autotune) times the candidates inside the captured
 // step loop and keeps the fastest combination.  Sizes without an entry use the
 // single rule-based schedule Sched<>.
@@ -27,14 +29,14 @@ struct StepCands {
   using type = SchedList<Sched<>>;
 };
 
-template <> struct StepCands<128, true, 1> { using type = SchedList<Sched<4, 4, 0, 1, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 0, 0>, Sched<8, 16, 1, 0, 0, 0>>; };  // 4.7 / 5.0 / 5.8 / 6.6 us
-template <> struct StepCands<128, true, 2> { using type = SchedList<Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 0>>; };  // 4.0 / 4.4 / 5.3 us
-template <> struct StepCands<128, true, 3> { using type = SchedList<Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 2, 0>, Sched<8, 8, 1, 0, 0, 0>>; };  // 3.6 / 3.9 / 4.5 / 4.6 us
-template <> struct StepCands<128, true, 4> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 0, 0, 2, 1>>; };  // 3.9 / 4.2 / 4.8 us
+template <> struct StepCands<128, true, 1> { using type = SchedList<Sched<4, 4, 0, 1, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 0, 0>, Sched<8, 16, 1, 0, 0, 0>, Sched<8, 16, 1, 0, 2, 0>>; };  // 4.7 / 5.0 / 5.8 / 6.6; 64 instances per launch: 8.7 us
+template <> struct StepCands<128, true, 2> { using type = SchedList<Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 1>>; };  // 4.0 / 4.4 / 5.3; 64 instances per launch: 9.8 us
+template <> struct StepCands<128, true, 3> { using type = SchedList<Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 2, 0>, Sched<8, 8, 1, 0, 0, 0>, Sched<8, 8, 1, 0, 2, 0>>; };  // 3.6 / 3.9 / 4.5 / 4.6; 64 instances per launch: 11.0 us
+template <> struct StepCands<128, true, 4> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 0, 0, 2, 1>, Sched<4, 2, 0, 0, 2, 0>>; };  // 3.9 / 4.2 / 4.8; 64 instances per launch: 7.5 us
 template <> struct StepCands<128, false, 1> { using type = SchedList<Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 0, 0>, Sched<4, 8, 1, 0, 0, 0>, Sched<4, 16, 1, 0, 2, 0>>; };  // 4.3 / 4.6 / 4.7 / 5.3 us
-template <> struct StepCands<128, false, 2> { using type = SchedList<Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 0>, Sched<4, 8, 0, 1, 0, 0>>; };  // 3.7 / 3.9 / 4.5 us
-template <> struct StepCands<128, false, 3> { using type = SchedList<Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 2, 0>, Sched<8, 16, 1, 0, 0, 0>>; };  // 3.3 / 3.5 / 3.9 / 4.3 us
-template <> struct StepCands<128, false, 4> { using type = SchedList<Sched<4, 2, 0, 0, 1, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 1>>; };  // 3.7 / 3.8 / 4.3 us
+template <> struct StepCands<128, false, 2> { using type = SchedList<Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 0>, Sched<4, 8, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 0, 1>>; };  // 3.7 / 3.9 / 4.5; 64 instances per launch: 6.8 us
+template <> struct StepCands<128, false, 3> { using type = SchedList<Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<4, 16, 1, 0, 2, 0>, Sched<8, 16, 1, 0, 0, 0>, Sched<8, 16, 1, 0, 2, 0>>; };  // 3.3 / 3.5 / 3.9 / 4.3; 64 instances per launch: 6.9 us
+template <> struct StepCands<128, false, 4> { using type = SchedList<Sched<4, 2, 0, 0, 1, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 1>, Sched<4, 2, 0, 0, 0, 0>>; };  // 3.7 / 3.8 / 4.3; 64 instances per launch: 6.5 us
 template <> struct StepCands<192, true, 1> { using type = SchedList<Sched<6, 4, 1, 1, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<6, 16, 1, 0, 2, 0>>; };  // 7.0 / 7.4 / 8.6 us
 template <> struct StepCands<192, true, 2> { using type = SchedList<Sched<6, 2, 0, 1, 0, 0>, Sched<6, 4, 0, 1, 2, 0>, Sched<12, 4, 0, 1, 0, 0>>; };  // 7.0 / 7.7 / 8.9 us
 template <> struct StepCands<192, true, 3> { using type = SchedList<Sched<6, 4, 0, 1, 0, 0>, Sched<6, 8, 0, 0, 2, 0>, Sched<6, 16, 1, 0, 2, 0>>; };  // 5.8 / 6.0 / 7.0 us
@@ -43,29 +45,29 @@ template <> struct StepCands<192, false, 1> { using type = SchedList<Sched<6, 4,
 template <> struct StepCands<192, false, 2> { using type = SchedList<Sched<6, 2, 0, 1, 0, 0>, Sched<6, 4, 0, 1, 0, 0>, Sched<12, 4, 0, 1, 0, 0>>; };  // 5.9 / 6.4 / 7.6 us
 template <> struct StepCands<192, false, 3> { using type = SchedList<Sched<6, 4, 1, 0, 0, 0>, Sched<6, 8, 0, 0, 0, 0>, Sched<6, 16, 1, 0, 2, 0>>; };  // 5.0 / 5.3 / 5.8 us
 template <> struct StepCands<192, false, 4> { using type = SchedList<Sched<6, 2, 0, 0, 0, 0>, Sched<6, 4, 0, 0, 0, 0>, Sched<6, 8, 0, 0, 1, 0>>; };  // 5.7 / 6.0 / 6.9 us
-template <> struct StepCands<256, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 0>>; };  // 5.3 / 5.5 / 6.3 us
+template <> struct StepCands<256, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 0>, Sched<4, 8, 0, 0, 2, 1>>; };  // 5.3 / 5.5 / 6.3; 64 instances per launch: 33.1 us
 template <> struct StepCands<256, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 0>>; };  // 4.3 / 4.6 / 5.7 / 6.0 us
-template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 16, 0, 0, 2, 1>>; };  // 4.3 / 4.4 / 5.0 / 6.6 us
+template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 16, 0, 0, 2, 1>, Sched<4, 8, 1, 0, 0, 1>>; };  // 4.3 / 4.4 / 5.0 / 6.6; 64 instances per launch: 35.2 us
 template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<4, 2, 0, 1, 2, 1>, Sched<4, 4, 0, 0, 1, 1>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.0 / 4.3 / 4.9 / 6.8 us
 template <> struct StepCands<256, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 4, 1, 0, 0, 0>, Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>>; };  // 4.8 / 5.0 / 5.0 / 5.5 us
-template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 1, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 0, 0>>; };  // 3.8 / 4.2 / 4.8 / 5.4 us
+template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 1, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 1>>; };  // 3.8 / 4.2 / 4.8 / 5.4; 64 instances per launch: 17.3 us
 template <> struct StepCands<256, false, 3> { using type = SchedList<Sched<4, 2, 1, 1, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 8, 1, 0, 2, 0>>; };  // 3.8 / 4.0 / 4.1 / 5.0 us
-template <> struct StepCands<256, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 0>, Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 1, 0>>; };  // 3.6 / 3.8 / 4.3 us
-template <> struct StepCands<384, true, 1> { using type = SchedList<Sched<6, 2, 1, 1, 0, 0>, Sched<6, 4, 1, 0, 2, 0>, Sched<6, 4, 0, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>>; };  // 8.0 / 8.1 / 8.2 / 9.4 us
-template <> struct StepCands<384, true, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 2, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>>; };  // 8.1 / 8.9 / 10.3 / 15.2 us
-template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 1, 0, 2, 0>, Sched<6, 2, 1, 1, 2, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 2, 0>>; };  // 6.9 / 7.4 / 8.0 / 8.3 us
+template <> struct StepCands<256, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 0>, Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 1, 0>, Sched<4, 2, 0, 0, 0, 1>>; };  // 3.6 / 3.8 / 4.3; 64 instances per launch: 15.5 us
+template <> struct StepCands<384, true, 1> { using type = SchedList<Sched<6, 2, 1, 1, 0, 0>, Sched<6, 4, 1, 0, 2, 0>, Sched<6, 4, 0, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<6, 8, 1, 0, 2, 1>>; };  // 8.0 / 8.1 / 8.2 / 9.4; 64 instances per launch: 74.3 us
+template <> struct StepCands<384, true, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 2, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>, Sched<12, 8, 0, 1, 0, 0>>; };  // 8.1 / 8.9 / 10.3 / 15.2; 64 instances per launch: 80.1 us
+template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 1, 0, 2, 0>, Sched<6, 2, 1, 1, 2, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 0, 0>>; };  // 6.9 / 7.4 / 8.0 / 8.3; 64 instances per launch: 104.2 us
 template <> struct StepCands<384, true, 4> { using type = SchedList<Sched<6, 1, 0, 1, 2, 0>, Sched<6, 2, 0, 0, 2, 0>, Sched<6, 4, 0, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>>; };  // 7.1 / 7.7 / 9.6 / 10.4 us
-template <> struct StepCands<384, false, 1> { using type = SchedList<Sched<6, 2, 0, 0, 2, 0>, Sched<6, 4, 1, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>>; };  // 7.1 / 7.2 / 7.9 us
+template <> struct StepCands<384, false, 1> { using type = SchedList<Sched<6, 2, 0, 0, 2, 0>, Sched<6, 4, 1, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<6, 8, 1, 0, 2, 1>>; };  // 7.1 / 7.2 / 7.9; 64 instances per launch: 46.6 us
 template <> struct StepCands<384, false, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 0, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>>; };  // 6.6 / 7.2 / 8.7 / 11.4 us
 template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0, 0, 0>, Sched<6, 2, 1, 1, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>>; };  // 6.0 / 6.2 / 6.4 / 6.9 us
 template <> struct StepCands<384, false, 4> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 0, 0, 0>, Sched<6, 4, 0, 0, 1, 0>, Sched<12, 2, 0, 0, 0, 0>>; };  // 6.1 / 6.6 / 7.6 / 8.6 us
-template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 0>, Sched<4, 1, 1, 0, 2, 0>, Sched<4, 4, 0, 1, 2, 1>, Sched<4, 4, 1, 0, 2, 1>>; };  // 6.2 / 6.4 / 6.8 / 6.8 us
-template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 2, 1>>; };  // 6.4 / 6.6 / 6.8 / 8.2 us
+template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 0>, Sched<4, 1, 1, 0, 2, 0>, Sched<4, 4, 0, 1, 2, 1>, Sched<4, 4, 1, 0, 2, 1>, Sched<4, 4, 0, 1, 0, 1>>; };  // 6.2 / 6.4 / 6.8 / 6.8; 64 instances per launch: 128.6 us
+template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 2, 1>, Sched<8, 4, 0, 1, 1, 0>>; };  // 6.4 / 6.6 / 6.8 / 8.2; 64 instances per launch: 108.2 us
 template <> struct StepCands<512, true, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 0>, Sched<8, 8, 0, 0, 0, 0>>; };  // 5.9 / 6.6 / 6.6 / 7.4 us
-template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<8, 2, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 1, 1>>; };  // 5.7 / 6.1 / 6.2 / 7.2 us
-template <> struct StepCands<512, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 1, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<8, 2, 0, 1, 0, 0>>; };  // 5.5 / 5.5 / 5.9 / 6.7 us
-template <> struct StepCands<512, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 1, 0>>; };  // 5.3 / 5.5 / 5.7 us
-template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 0>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>, Sched<8, 16, 0, 1, 0, 1>>; };  // 4.9 / 5.2 / 5.4 / 7.2 us
+template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<8, 2, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 1, 1>, Sched<8, 2, 0, 1, 0, 0>>; };  // 5.7 / 6.1 / 6.2 / 7.2; 64 instances per launch: 94.8 us
+template <> struct StepCands<512, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 1, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 2, 0>>; };  // 5.5 / 5.5 / 5.9 / 6.7; 64 instances per launch: 66.2 us
+template <> struct StepCands<512, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 1, 0>, Sched<8, 1, 0, 1, 1, 0>>; };  // 5.3 / 5.5 / 5.7; 64 instances per launch: 59.2 us
+template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 0>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>, Sched<8, 16, 0, 1, 0, 1>, Sched<8, 16, 0, 1, 0, 0>>; };  // 4.9 / 5.2 / 5.4 / 7.2; 64 instances per launch: 77.3 us
 template <> struct StepCands<512, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<8, 2, 0, 0, 0, 0>>; };  // 4.8 / 5.1 / 5.5 us
 template <> struct StepCands<768, true, 1> { using type = SchedList<Sched<6, 4, 1, 0, 2, 1>, Sched<6, 1, 1, 0, 2, 0>, Sched<6, 2, 1, 0, 2, 0>>; };  // 11.3 / 11.8 / 12.1 us
 template <> struct StepCands<768, true, 2> { using type = SchedList<Sched<12, 2, 0, 1, 0, 0>, Sched<12, 1, 0, 1, 0, 0>, Sched<6, 1, 0, 1, 2, 0>>; };  // 12.9 / 13.4 / 14.6 us

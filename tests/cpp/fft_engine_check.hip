@@ -170,6 +170,9 @@ int main() {
   // other register tiles (elements per thread) the pass schedules may use
   bad += check<512, double, 1024, 4>(1e-14);
   bad += check<1024, double, 1024, 4>(1e-14);
+  bad += check<1024, double, 1024, 2>(1e-14);
+  bad += check<512, double, 1024, 2>(1e-14);
+  bad += check<1024, float, 1024, 2>(2e-6);
   bad += check<1024, double, 1024, 16>(1e-14);
   bad += check<2048, double, 4096, 16>(1e-14);
   bad += check<256, double, 512, 4>(1e-14);

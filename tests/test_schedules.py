@@ -35,14 +35,14 @@ def test_schedule_cache_roundtrip(pkg, tmp_path, monkeypatch):
 # ---- GPU -------------------------------------------------------------------
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [128, 192, 512, 1024])
+@pytest.mark.parametrize("n", [128, 192, 256, 384, 512, 768, 1024])
 def test_every_candidate_matches_oracle_fp64(pkg, n):
     c, phi = _fields(pkg, n)
     steps = 10
     cr, hr = o.run_simulation_fast(c, phi, steps=steps)
     s = pkg.Session(n)
     info = s.schedule_info()
-    assert [len(x) for x in info] and all(1 <= len(x) <= 4 for x in info)
+    assert [len(x) for x in info] and all(1 <= len(x) <= 5 for x in info)
     for q in range(4):
         for i, sc in enumerate(info[q]):
             sel = [0, 0, 0, 0]
@@ -58,7 +58,7 @@ def test_every_candidate_matches_oracle_fp64(pkg, n):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [256, 384, 1024])
+@pytest.mark.parametrize("n", [128, 256, 384, 512, 1024])
 def test_every_candidate_mixed_within_tolerance(pkg, n):
     """Mixed form: FP32 bulk work, so each candidate is held to the mixed-form
     tolerance against the FP64 oracle over 10 steps."""
