@@ -79,6 +79,33 @@ def test_every_candidate_mixed_within_tolerance(pkg, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n, precision, tol", [(2048, "fp64", 1e-12), (4096, "fp64", 1e-12),
+                                               (2048, "mixed", 5e-6), (4096, "mixed", 5e-6)])
+def test_large_candidates_agree_with_default(pkg, n, precision, tol):
+    """At 2048^2 and 4096^2 (too large for a quick oracle run of every
+    candidate) each compiled candidate must reproduce the default schedule's
+    field after 3 steps; the default itself is held to the oracle by the
+    large-field tests.  FP64 to 1e-12; mixed to its own tolerance, since the
+    FP32 bulk rounds differently per radix plan and exchange order."""
+    c, phi = _fields(pkg, n)
+    s = pkg.Session(n, precision=precision)
+    s.set_fields(c, phi)
+    s.run(0.01, 3)
+    ref = s.field()
+    info = s.schedule_info()
+    for q in range(4):
+        for i in range(1, len(info[q])):
+            sel = [0, 0, 0, 0]
+            sel[q] = i
+            say(f"n={n} {precision} pass {q} candidate {i} {info[q][i]}")
+            s.set_schedule(sel)
+            s.set_fields(c, phi)
+            s.run(0.01, 3)
+            assert np.max(np.abs(s.field() - ref)) < tol
+    s.close()
+
+
+@pytest.mark.gpu
 def test_set_schedule_rejects_bad_index(pkg):
     s = pkg.Session(64)
     with pytest.raises(pkg.CsgError):
