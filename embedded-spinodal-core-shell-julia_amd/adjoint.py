@@ -5,7 +5,9 @@
 with c(T) the result of `steps` applications of step (core_shell.jl:72-76)
 from c0.  Returns L, dL/dc0 and dL/d(kappa, P, chi).  The backward sweep runs
 on the GPU (csg_adjoint_forward / csg_adjoint_backward) with checkpoints every
-`ckpt_every` steps (default ~sqrt(steps)); the loss's direct parameter terms
+`ckpt_every` steps.  By default every state is kept when the run's states fit
+KEEP_ALL_BYTES of device memory (the backward sweep then recomputes nothing),
+else one checkpoint per ~sqrt(steps) steps; the loss's direct parameter terms
 lam dF/dtheta use that F is affine in kappa, P and chi, evaluated with the
 GPU energy.  FP64 only.
 """
@@ -17,6 +19,19 @@ import math
 import numpy as np
 
 from ._capi import check, lib, ptr
+
+# Device memory the default checkpointing may spend on keeping every state of
+# the forward run: (steps + 1) FP64 fields of N^2 values (2000 steps of 512^2
+# take 4.2 GB).
+KEEP_ALL_BYTES = 32 << 30
+
+
+def default_ckpt_every(n: int, steps: int) -> int:
+    """1 (keep every state) when (steps + 1) N^2 doubles fit KEEP_ALL_BYTES,
+    else ~sqrt(steps)."""
+    if (steps + 1) * n * n * 8 <= KEEP_ALL_BYTES:
+        return 1
+    return max(1, int(round(math.sqrt(steps))))
 from .core_shell import Parameters, Session, _fortran
 
 
@@ -73,7 +88,7 @@ def gradient(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float
              device: int = 0) -> Gradient:
     c0, phi, target = _fortran(c0), _fortran(phi), _fortran(target)
     n = c0.shape[0]
-    K = ckpt_every or max(1, int(round(math.sqrt(steps))))
+    K = ckpt_every or default_ckpt_every(n, steps)
     s = _context(n, spacing, p, device)
     s.set_fields(c0, phi)
     check(lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K)))

@@ -226,12 +226,23 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   if (threadIdx.x == 0) a.acc[blockIdx.x] += rr[0] * 2.0 * dt / ((double)N * N);
 }
 
+// Rows per workgroup of the adjoint row pass: the forward rule for two
+// arrays, halved while the grid has fewer workgroups than MI355X has CUs
+// (256) and the workgroup stays whole wavefronts (at 512^2 the rule alone
+// gives 4 rows and 128 workgroups, half the chip idle).
+template <int N, int E>
+constexpr int adj_rows() {
+  int b = row_rows<N, double, 2, E>();
+  while (b > 1 && N / b < 256 && (b / 2) * (N / 2 / E) >= 64) b /= 2;
+  return b;
+}
+
 template <int N>
 struct AdjRowCfg {
   static constexpr int L = N / 2;
   static constexpr int E = adj_E(L);
   static constexpr int TPL = L / E;
-  static constexpr int B = row_rows<N, double, 2, E>();
+  static constexpr int B = adj_rows<N, E>();
   static constexpr int LP = padded_len<double>(L);
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;

@@ -67,6 +67,11 @@ struct csg_ctx {
   // captured full-segment graphs of the adjoint: forward steps, recompute,
   // backward sweep (valid for adj_gK steps of adj_gdt with the current kernels)
   hipGraphExec_t adj_fwd = nullptr, adj_rec = nullptr, adj_bwd = nullptr;
+  // every state kept (ckpt_every = 1): forward and backward graphs per block
+  // of STEPS_PER_GRAPH steps, bound to the checkpoint buffers of a run of
+  // adj_all_T steps
+  std::vector<hipGraphExec_t> adj_all_fwd, adj_all_bwd;
+  int64_t adj_all_T = 0;
   int64_t adj_gK = 0;
   double adj_gdt = 0.0;
   DevParams adj_gp{};     // parameters captured in the adjoint graphs
@@ -681,6 +686,12 @@ void drop_adjoint_graphs(csg_ctx* c) {
     if (*g) (void)hipGraphExecDestroy(*g);
     *g = nullptr;
   }
+  for (std::vector<hipGraphExec_t>* v : {&c->adj_all_fwd, &c->adj_all_bwd}) {
+    for (hipGraphExec_t g : *v)
+      if (g) (void)hipGraphExecDestroy(g);
+    v->clear();
+  }
+  c->adj_all_T = 0;
   c->adj_gK = 0;
 }
 
@@ -713,23 +724,27 @@ AdjArgs adjoint_args(const csg_ctx* c) {
   return a;
 }
 
-// Backward sweep over steps m-1 .. 0 of a segment whose states are seg[0..m].
-void enqueue_backward_segment(csg_ctx* c, int64_t m) {
+// Backward step through the forward step S0 -> S1: gbar (c->G) becomes
+// dL/dS0, the parameter terms accumulate in c->adj_acc.
+void enqueue_backward_step(csg_ctx* c, const void* S0, const void* S1) {
   AdjArgs a = adjoint_args(c);
   const int na = c->ops.n_adj_col;
-  for (int64_t i = m - 1; i >= 0; --i) {
-    a.S0 = static_cast<const cplx<double>*>(c->seg[i]);
-    a.S1 = static_cast<const cplx<double>*>(c->seg[i + 1]);
-    a.acc = c->adj_acc;
-    c->ops.adj1(a, c->tw, c->stream);
-    PassArgs pa = base_args(c);
-    pa.in0 = c->A; pa.in1 = c->Bb; pa.out0 = c->A; pa.out1 = c->Bb; pa.part = part3(c);
-    c->ops.p3(pa, c->tw, c->stream);
-    a.acc = c->adj_acc + na;
-    c->ops.adj3(a, c->tw, c->stream);
-    a.acc = c->adj_acc + 2 * na;
-    c->ops.adj4(a, c->tw, c->stream);
-  }
+  a.S0 = static_cast<const cplx<double>*>(S0);
+  a.S1 = static_cast<const cplx<double>*>(S1);
+  a.acc = c->adj_acc;
+  c->ops.adj1(a, c->tw, c->stream);
+  PassArgs pa = base_args(c);
+  pa.in0 = c->A; pa.in1 = c->Bb; pa.out0 = c->A; pa.out1 = c->Bb; pa.part = part3(c);
+  c->ops.p3(pa, c->tw, c->stream);
+  a.acc = c->adj_acc + na;
+  c->ops.adj3(a, c->tw, c->stream);
+  a.acc = c->adj_acc + 2 * na;
+  c->ops.adj4(a, c->tw, c->stream);
+}
+
+// Backward sweep over steps m-1 .. 0 of a segment whose states are seg[0..m].
+void enqueue_backward_segment(csg_ctx* c, int64_t m) {
+  for (int64_t i = m - 1; i >= 0; --i) enqueue_backward_step(c, c->seg[i], c->seg[i + 1]);
 }
 
 // Recompute seg[1..m] from seg[0].
@@ -752,7 +767,36 @@ int csg_adjoint_forward(csg_ctx* c, double dt, int64_t steps, int64_t ckpt_every
   const size_t nck = static_cast<size_t>(nfull) + 1;
   if (int rc = ensure_bufs(c, c->ckpt, nck, bytes)) return rc;
   c->adj_dt = dt;
-  if (c->adj_gK != K || c->adj_gdt != dt || !same_params(c->adj_gp, c->dp)) drop_adjoint_graphs(c);
+  if (c->adj_gK != K || c->adj_gdt != dt || !same_params(c->adj_gp, c->dp) || (K == 1 && c->adj_all_T != steps))
+    drop_adjoint_graphs(c);
+  if (K == 1) {
+    // every state kept: each step writes its result straight into the next
+    // checkpoint (out-of-place P4), so the backward sweep needs no recompute
+    const int64_t nb = (steps + STEPS_PER_GRAPH - 1) / STEPS_PER_GRAPH;
+    if (c->adj_all_fwd.empty()) {
+      for (int64_t bk = 0; bk < nb; ++bk) {
+        hipGraphExec_t g = nullptr;
+        const int64_t s0 = bk * STEPS_PER_GRAPH, s1 = std::min<int64_t>(s0 + STEPS_PER_GRAPH, steps);
+        if (int rc = capture_graph(c, &g, [&]() {
+              for (int64_t s = s0; s < s1; ++s) enqueue_plain_step(c, c->ckpt[s], c->ckpt[s + 1], dt);
+            }))
+          return rc;
+        c->adj_all_fwd.push_back(g);
+      }
+      c->adj_all_T = steps;
+      c->adj_gK = 1;
+      c->adj_gdt = dt;
+      c->adj_gp = c->dp;
+    }
+    HIPCHK(hipMemcpyAsync(c->ckpt[0], c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+    for (hipGraphExec_t g : c->adj_all_fwd) HIPCHK(hipGraphLaunch(g, c->stream));
+    HIPCHK(hipMemcpyAsync(c->S, c->ckpt[steps], bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->adj_steps = steps;
+    c->adj_every = 1;
+    return CSG_OK;
+  }
   // full segments replay one captured graph of K steps when there are several
   if (nfull >= 2 && !c->adj_fwd) {
     if (int rc = capture_graph(c, &c->adj_fwd, [&]() {
@@ -793,13 +837,39 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
   const int na = c->ops.n_adj_col, nr = c->ops.n_adj_row;
   const size_t nacc = 2 * static_cast<size_t>(na) + nr;
   if (!c->adj_acc && dmalloc(c, reinterpret_cast<void**>(&c->adj_acc), nacc * sizeof(double))) return CSG_ERR_HIP;
-  if (int rc = ensure_bufs(c, c->seg, static_cast<size_t>(K) + 1, bytes)) return rc;
+  if (K > 1)
+    if (int rc = ensure_bufs(c, c->seg, static_cast<size_t>(K) + 1, bytes)) return rc;
   HIPCHK(hipMemsetAsync(c->adj_acc, 0, nacc * sizeof(double), c->stream));
   // gbar_T into the state's packed line layout
   if (int rc = upload_real(c, dL_dcT)) return rc;
   if (int rc = set_state_from_real(c, c->G)) return rc;
   const int64_t nseg = (T + K - 1) / K, nfull = T / K;
   (void)dt;
+  if (K == 1) {
+    // every state kept: step i's adjoint reads checkpoints i and i + 1.  The
+    // block graphs bind those buffers, dt and the parameters; any change of
+    // them, of the schedule or of T drops them (drop_adjoint_graphs), so
+    // graphs that exist here were captured for this run.
+    const int64_t nb = (T + STEPS_PER_GRAPH - 1) / STEPS_PER_GRAPH;
+    if (!c->adj_all_bwd.empty() && (c->adj_all_T != T || c->adj_gdt != c->adj_dt || !same_params(c->adj_gp, c->dp)))
+      drop_adjoint_graphs(c);
+    if (c->adj_all_bwd.empty()) {
+      c->adj_all_T = T;
+      c->adj_gK = 1;
+      c->adj_gdt = c->adj_dt;
+      c->adj_gp = c->dp;
+      for (int64_t bk = 0; bk < nb; ++bk) {
+        hipGraphExec_t g = nullptr;
+        const int64_t s0 = bk * STEPS_PER_GRAPH, s1 = std::min<int64_t>(s0 + STEPS_PER_GRAPH, T);
+        if (int rc = capture_graph(c, &g, [&]() {
+              for (int64_t i = s1 - 1; i >= s0; --i) enqueue_backward_step(c, c->ckpt[i], c->ckpt[i + 1]);
+            }))
+          return rc;
+        c->adj_all_bwd.push_back(g);
+      }
+    }
+    for (int64_t bk = nb - 1; bk >= 0; --bk) HIPCHK(hipGraphLaunch(c->adj_all_bwd[bk], c->stream));
+  } else {
   // full segments replay captured recompute and backward graphs when there are several
   if (c->adj_gK && (c->adj_gK != K || c->adj_gdt != c->adj_dt || !same_params(c->adj_gp, c->dp)))
     drop_adjoint_graphs(c);
@@ -825,6 +895,7 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
       enqueue_recompute_segment(c, m);
       enqueue_backward_segment(c, m);
     }
+  }
   }
   HIPCHK(hipGetLastError());
   std::vector<double> acc(nacc);
