@@ -130,15 +130,18 @@ int csg_time_passes(csg_ctx* ctx, double dt, int64_t nsteps, double* ms);
  * (core_shell.jl:72-76) iterated from the resident field.
  *
  * csg_adjoint_forward: advance the resident field by `steps` steps of `dt`,
- * keeping checkpoints every `ckpt_every` steps (sqrt(steps) is the usual
- * choice); the field c(T) stays resident. */
+ * keeping checkpoints every `ckpt_every` steps; the field c(T) stays
+ * resident.  ckpt_every = 1 keeps every state ((steps + 1) N^2 doubles of
+ * device memory), so the backward sweep recomputes nothing; larger
+ * intervals (sqrt(steps) minimises memory) trade memory for recomputation. */
 int csg_adjoint_forward(csg_ctx* ctx, double dt, int64_t steps, int64_t ckpt_every);
 
 /* csg_adjoint_backward: given dL/dc(T) (host, column-major N x N), return
  * dL/dc(0) (may be NULL) and dL_dparams[0..2] = the gradient with respect to
  * (kappa, P, chi) through the dynamics (may be NULL); the loss's own direct
  * dependence on the parameters is the caller's.  Recomputes each segment's
- * states from its checkpoint; the resident field is not changed. */
+ * states from its checkpoint (none when every state was kept); the resident
+ * field is not changed.  The result does not depend on ckpt_every. */
 int csg_adjoint_backward(csg_ctx* ctx, const double* dL_dcT, double* dL_dc0, double* dL_dparams);
 
 /* Bytes of device memory held by the context. */
