@@ -5,13 +5,20 @@
 #
 # where c(T) is `steps` applications of step (core_shell.jl:72-76) from c0
 # with Parameters `base` except kappa, P, chi.  The pullback runs the
-# hand-written adjoint on the GPU (csg_adjoint_forward / csg_adjoint_backward,
+# hand-written adjoint on the GPU (csg_adjoint_forward / csg_adjoint_backward;
+# every state kept when the run fits KEEP_ALL_BYTES of device memory, else
 # sqrt(steps) checkpoints) and returns cotangents for c0, kappa, P and chi.
 module CoreShellHIPChainRules
 
 using ChainRulesCore
 using ..CoreShellHIP: Parameters, Context, set_fields!, field, check, libcsg, energy, geometry,
                       make_grid, chemical_potential
+
+# Device memory the pullback may spend on keeping every state of the forward
+# run ((steps + 1) FP64 fields of N^2 values), as adjoint.py does.
+const KEEP_ALL_BYTES = 32 * 2^30
+
+default_every(n, steps) = (steps + 1) * n * n * 8 <= KEEP_ALL_BYTES ? 1 : max(1, round(Int, sqrt(steps)))
 
 function forward_ckpt(c0, phi, p, dt, steps, every, spacing)
     ctx = Context(size(c0, 1), spacing, p, :fp64, 0)
@@ -32,7 +39,7 @@ end
 function ChainRulesCore.rrule(::typeof(final_loss), c0, kappa, P, chi; phi, target, lam=0.0, dt=0.01,
                               steps=1000, spacing=1.0, base=Parameters())
     p = Parameters(base.A, base.B, chi, P, base.ca, base.cb, base.cc, kappa, base.mobility, base.Q)
-    ctx, cT = forward_ckpt(c0, phi, p, dt, steps, max(1, round(Int, sqrt(steps))), spacing)
+    ctx, cT = forward_ckpt(c0, phi, p, dt, steps, default_every(size(c0, 1), steps), spacing)
     shape, grid = geometry(phi, p), make_grid(size(c0, 1), spacing)
     L = 0.5 * sum(abs2, cT .- target) + lam * energy(cT, shape, grid, p)
     function pullback(dL)
