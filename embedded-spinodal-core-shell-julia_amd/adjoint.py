@@ -18,7 +18,7 @@ import math
 
 import numpy as np
 
-from ._capi import check, lib, ptr
+from ._capi import CSG_ERR_HIP, check, lib, ptr
 
 # Device memory the default checkpointing may spend on keeping every state of
 # the forward run: (steps + 1) FP64 fields of N^2 values (2000 steps of 512^2
@@ -91,7 +91,14 @@ def gradient(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float
     K = ckpt_every or default_ckpt_every(n, steps)
     s = _context(n, spacing, p, device)
     s.set_fields(c0, phi)
-    check(lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K)))
+    rc = lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K))
+    if rc == CSG_ERR_HIP and ckpt_every is None and K == 1:
+        # not enough free device memory to keep every state: checkpoint instead
+        # (buffers already allocated are kept and reused)
+        K = max(1, int(round(math.sqrt(steps))))
+        s.set_fields(c0, phi)
+        rc = lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K))
+    check(rc)
     cT = s.field()
     L = 0.5 * float(np.sum((cT - target) ** 2))
     gbar = cT - target

@@ -68,7 +68,12 @@ struct csg_batch {
 namespace {
 
 int bmalloc(csg_batch* b, void** p, size_t bytes) {
-  HIPCHK(hipMalloc(p, bytes));
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();   // leave no stale error for later checks
+    *p = nullptr;
+    return fail(CSG_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
   b->bytes += bytes;
   return CSG_OK;
 }

@@ -80,7 +80,14 @@ struct csg_ctx {
 namespace {
 
 int dmalloc(csg_ctx* c, void** p, size_t bytes) {
-  HIPCHK(hipMalloc(p, bytes));
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    // clear the runtime's last error, so a caller that falls back to a
+    // smaller allocation is not failed by a later hipGetLastError check
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(CSG_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
   c->bytes += bytes;
   return CSG_OK;
 }

@@ -129,3 +129,15 @@ def test_make_grid_matches_oracle(pkg):
         g = pkg.make_grid(n, sp)
         r = o.make_grid(n, sp)
         assert np.array_equal(g.kx, r["kx"]) and np.array_equal(g.ky, r["ky"]) and np.array_equal(g.k2, r["k2"])
+
+
+def test_adjoint_default_checkpointing(pkg):
+    """Every state is kept when the run's states fit the memory budget (the
+    512^2 x 2000-step workload: 4.2 GB), else ~sqrt(steps) checkpoints."""
+    assert pkg.adjoint.default_ckpt_every(512, 2000) == 1
+    assert pkg.adjoint.default_ckpt_every(4096, 2000) == 45      # 2001 x 128 MiB > 32 GiB
+    budget = pkg.adjoint.KEEP_ALL_BYTES
+    n = 1024
+    fit = budget // (n * n * 8) - 1          # largest step count whose states fit
+    assert pkg.adjoint.default_ckpt_every(n, fit) == 1
+    assert pkg.adjoint.default_ckpt_every(n, fit + 1) == max(1, round((fit + 1) ** 0.5))
