@@ -113,10 +113,17 @@ def run_form(pkg, form, c0, phi, a):
     hist = s.run(a.dt, a.steps)               # synchronous: returns after the stream drains
     t1 = time.perf_counter()
     c = s.field()
+    # five shorter repeats (continuing from the field) for a median and spread
+    reps = []
+    for _ in range(5):
+        k = max(1, a.steps // 10)
+        r0 = time.perf_counter()
+        s.run(a.dt, k)
+        reps.append(1e3 * (time.perf_counter() - r0) / (k + 1))
     ms_pass = s.time_passes(a.dt, 50)
     sched = list(s.schedule)
     s.close()
-    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, tune=tune)
+    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, tune=tune, reps=reps)
 
 
 def gradient_rate(pkg, a):
@@ -297,6 +304,20 @@ def cpu_baseline(c0, phi, a, c_gpu_steps=None):
                           f"{a.cpu_steps + 1} loop iterations of the {a.n}^2 workload in {secs:.1f} s")
 
 
+def gpu_clocks():
+    """The GPU's current clock levels as rocm-smi reports them (None if the
+    tool is not available)."""
+    import subprocess
+    try:
+        r = subprocess.run(["rocm-smi", "--showclocks"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=30)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    keep = [ln.split(":", 1)[-1].strip() if "GPU[" in ln else ln.strip()
+            for ln in r.stdout.splitlines() if "clock level" in ln.lower()]
+    return keep or None
+
+
 def main():
     a = _args()
     if a.calibrate_copy:
@@ -329,7 +350,9 @@ def main():
                  dominant_bytes=PASS_BYTES[f][dom] * sites,
                  energy_final=float(r["hist"][-1, 2]),
                  mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))),
-                 schedule=r["schedule"], autotune=r["tune"])
+                 schedule=r["schedule"], autotune=r["tune"],
+                 ms_per_step_repeats=[float(x) for x in r["reps"]],
+                 ms_per_step_median_of_repeats=float(np.median(r["reps"])))
         if ref is not None and f != "fp64":
             o["max_dc_vs_fp64"] = float(np.max(np.abs(r["c"] - ref["c"])))
             o["max_dc_vs_fp64_1000_steps"] = short_horizon_error(pkg, f, c0, phi, a)
@@ -355,6 +378,7 @@ def main():
                    "form": pick, "grid": a.n},
         "roofline": roof,
         "forms": out,
+        "gpu_clocks": gpu_clocks(),
     }
     if a.ens_count:
         line["ensemble"] = ensemble_rate(pkg, a)
