@@ -166,3 +166,33 @@ def test_mixed_context_rejects_adjoint(pkg):
     s.set_fields(c0, phi)
     assert pkg.lib().csg_adjoint_forward(s.handle, 0.01, 10, 3) == _capi.CSG_ERR_UNSUPPORTED
     s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_kept_states_survive_schedule_change(pkg):
+    """With every state kept (ckpt_every = 1), a schedule change between the
+    forward and the backward sweep drops the captured graphs but not the
+    states: the backward still runs and gives the gradient of the same run
+    (its own P3 launches now use the other schedule, so to rounding)."""
+    import ctypes
+    from csg_amd import _capi
+    n, steps, dt = 128, 60, 0.01
+    c0, phi = pkg.initial_fields(n, radius=38.4, width=2.0)
+    c0, phi = np.asarray(c0), np.asarray(phi)
+    rng = np.random.default_rng(3)
+    gbar = np.asfortranarray(rng.standard_normal((n, n)))
+    out = []
+    for change in (False, True):
+        s = pkg.Session(n)
+        s.set_fields(c0, phi)
+        assert pkg.lib().csg_adjoint_forward(s.handle, dt, steps, 1) == 0
+        if change:
+            s.set_schedule((0, 0, 1, 0))
+        dc0 = np.empty((n, n), order="F")
+        dp = (ctypes.c_double * 3)()
+        _capi.check(pkg.lib().csg_adjoint_backward(s.handle, _capi.ptr(gbar), _capi.ptr(dc0), dp))
+        out.append((dc0, np.array(dp[:])))
+        s.close()
+    assert np.max(np.abs(out[1][0] - out[0][0])) <= 1e-10 * np.max(np.abs(out[0][0]))
+    assert np.all(np.abs(out[1][1] - out[0][1]) <= 1e-10 * np.abs(out[0][1]))
+
