@@ -260,11 +260,12 @@ constexpr int stab_exp_rt(int len, int e, bool inv, int q) {
 
 // Twiddle buffer of an N-point grid: W_N^m for m in [0, N), then the stage
 // tables (forward plan, and the inverse plan's when it differs) of every
-// (LEN, E) a kernel may use, LEN in {N, N/2}, E in stab_e_list.  Kernels copy
-// their tables from the buffer to LDS with contiguous loads.
-constexpr int stab_e_count = 9;
+// (LEN, E) a kernel may use: LEN in {N, N/2} and every valid E of the
+// stab_e_at list.  Kernels copy their tables from the buffer to LDS with
+// contiguous loads.
+constexpr int stab_e_count = 8;
 constexpr int stab_e_at(int i) {
-  constexpr int es[stab_e_count] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
+  constexpr int es[stab_e_count] = {2, 4, 6, 8, 12, 16, 24, 32};
   return es[i];
 }
 constexpr bool stab_e_ok(int len, int e) { return len >= 2 && fft_E_valid(len, e); }
