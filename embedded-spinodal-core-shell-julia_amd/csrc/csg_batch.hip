@@ -26,6 +26,7 @@ namespace {
 struct BatchClass {
   int n = 0;
   int m = 0;                     // instances
+  int nt = 0;                    // non-temporal pass outputs (the ensemble's choice)
   Ops ops{};
   std::vector<int> members;      // batch index of slab slot q
   std::vector<DevParams> hparams;
@@ -63,6 +64,7 @@ struct csg_batch {
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
   size_t bytes = 0;
+  int nt = 0;   // non-temporal pass outputs (nontemporal_for the ensemble's footprint)
 };
 
 namespace {
@@ -81,6 +83,7 @@ int bmalloc(csg_batch* b, void** p, size_t bytes) {
 // Pass arguments covering all instances of class k.
 PassArgs class_args(const BatchClass& k) {
   PassArgs a{};
+  a.nt = k.nt;
   a.p = k.hparams[0];
   a.phi = k.phi;
   a.ninst = k.m;
@@ -305,6 +308,8 @@ int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const dou
       if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
     }
   }
+  b->nt = nontemporal_for(b->bytes);
+  for (BatchClass& k : b->cls) k.nt = b->nt;
   *out = b;
   return CSG_OK;
 }

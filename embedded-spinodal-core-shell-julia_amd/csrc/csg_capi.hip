@@ -56,6 +56,7 @@ struct csg_ctx {
   double gdt = 0.0;
   DevParams gp{};         // parameters captured in gexec
   size_t bytes = 0;
+  int nt = 0;             // non-temporal pass outputs (nontemporal_for the footprint at creation)
   // adjoint: checkpoints of the forward run, segment states, adjoint field
   std::vector<void*> ckpt;
   std::vector<void*> seg;
@@ -96,6 +97,7 @@ PassArgs base_args(const csg_ctx* c) {
   PassArgs a{};
   a.p = c->dp;
   a.phi = c->phi;
+  a.nt = c->nt;
   return a;
 }
 
@@ -309,6 +311,7 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
     e = hipMemcpy(c->tw, tw32.data(), tw32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
   }
+  c->nt = nontemporal_for(c->bytes);
   *out = c;
   return CSG_OK;
 }

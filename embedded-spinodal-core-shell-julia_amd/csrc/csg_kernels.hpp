@@ -90,6 +90,7 @@ struct PassArgs {
   InstanceStrides is;
   int ninst;         // instances (gridDim.y); 0 or 1: a single field
   const void* sin;   // P4: state read from here and the sum written to out0 (NULL: in place)
+  int nt;            // 1: the step passes store their outputs non-temporally
 };
 
 __device__ __forceinline__ const void* offset_bytes(const void* p, long long q, long long s) {
@@ -155,6 +156,22 @@ constexpr int tw_lds_len() { return tw_in_lds<N, CT>() ? N / 2 : 1; }
 // LDS-resident and the tables of the (LEN, E) plans fit 32 KiB.
 template <int N, class CT, int LEN, int E>
 constexpr bool stab_fits() { return tw_in_lds<N, CT>() && stab_len(LEN, E) * 2 * (int)sizeof(CT) <= 32 * 1024; }
+
+// Store a step pass output.  nt: non-temporal, so the line is not kept in
+// this XCD's L2; the next pass reads it from other XCDs anyway.  Pays while
+// the step's working set stays in the Infinity Cache (the host decides).
+template <class T>
+__device__ __forceinline__ void store_out(cplx<T>* p, cplx<T> v, int nt) {
+  if (nt) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
   const double t = s + v;
@@ -477,7 +494,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     for (int r = 0; r < E; ++r) {
       const int f = threadIdx.x + NT * r;
       const int bb = f % B, kx = f / B;
-      X[(size_t)kx * N + j0 + bb] = cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT>(kx)]);
+      store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT>(kx)]), a.nt);
     }
   }
 
@@ -651,8 +668,8 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       cplx<CT>* O1 = static_cast<cplx<CT>*>(a.out1);
 #pragma unroll
       for (int j = 0; j < E; ++j) {
-        O0[base + t + TPL * j] = v[0][j];
-        O1[base + t + TPL * j] = v[1][j];
+        store_out(&O0[base + t + TPL * j], v[0][j], a.nt);
+        store_out(&O1[base + t + TPL * j], v[1][j], a.nt);
       }
       double r[1] = {esum};
       const int ops[1] = {RED_SUM};
@@ -727,7 +744,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
         cplx<TS> s = sp[j];
         s.x += static_cast<TS>(w[0][j].x);
         s.y += static_cast<TS>(w[0][j].y);
-        S[base + t + TPL * j] = s;
+        store_out(&S[base + t + TPL * j], s, a.nt);
       }
 
     }

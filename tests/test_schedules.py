@@ -183,3 +183,30 @@ def test_ensemble_schedules_follow_single_sessions(pkg):
     for i in range(3):
         assert np.array_equal(ens.field(i), before[i])
     ens.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nontemporal_outputs_are_bit_identical(pkg, monkeypatch, precision):
+    """The non-temporal store hint (chosen from the device footprint, forced
+    here with CSG_NONTEMPORAL) changes where pass outputs are cached, never
+    their values: fields and histories agree bit for bit, single fields and
+    ensembles alike."""
+    n, steps = 256, 40
+    c, phi = _fields(pkg, n)
+    out = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("CSG_NONTEMPORAL", v)
+        s = pkg.Session(n, precision=precision)
+        s.set_fields(c, phi)
+        h = s.run(0.01, steps)
+        ens = pkg.Ensemble([n, 128], precision=precision)
+        ens.set_fields(0, c, phi)
+        c1, phi1 = _fields(pkg, 128)
+        ens.set_fields(1, c1, phi1)
+        ens.run(0.01, steps)
+        out[v] = (s.field(), h, ens.field(0), ens.field(1))
+        s.close()
+        ens.close()
+    for a, b in zip(out["0"], out["1"]):
+        assert np.array_equal(a, b)
