@@ -64,17 +64,20 @@ std::vector<cplx<T>> make_twiddles(int n) {
   return w;
 }
 
+// Cache hints of the step passes (NT_STORE / NT_LOAD) by device footprint.
 // Non-temporal pass outputs pay while a step's working set stays well inside
-// the 256 MB Infinity Cache and cost beyond (captured loop, by device
-// footprint: 1024^2 FP64 40 MiB -5 %, mixed 28 MiB -6 %; 2048^2 mixed
-// 113 MiB +6 %, FP64 161 MiB +22 %; 4096^2 and the 256-particle ensemble,
-// 450-950 MiB, +3 to +19 %).  A context or ensemble uses them when its device
-// footprint is at most NT_MAX_BYTES; CSG_NONTEMPORAL=0 or 1 overrides.
+// the 256 MB Infinity Cache and cost beyond it (captured loop: 1024^2 FP64
+// 40 MiB -5 %, mixed 28 MiB -6 %; 2048^2 mixed 113 MiB +6 %, FP64 161 MiB
+// +22 %; 4096^2 and the 256-particle ensemble, 450-950 MiB, +3 to +19 %).
+// Non-temporal column-pass loads are the reverse: 4096^2 mixed -6 %, FP64 and
+// ensembles -1 %, 1024^2 mixed +2 %.  A context or ensemble with a footprint
+// of at most NT_MAX_BYTES streams its stores, a larger one its column loads.
+// CSG_NONTEMPORAL=0..3 (the hint bits) overrides.
 constexpr size_t NT_MAX_BYTES = size_t(64) << 20;
 inline int nontemporal_for(size_t bytes) {
   if (const char* e = std::getenv("CSG_NONTEMPORAL"))
-    if (e[0] == '0' || e[0] == '1') return e[0] == '1';
-  return bytes <= NT_MAX_BYTES ? 1 : 0;
+    if (e[0] >= '0' && e[0] <= '3' && e[1] == '\0') return e[0] - '0';
+  return bytes <= NT_MAX_BYTES ? NT_STORE : NT_LOAD;
 }
 
 constexpr int FIN_NT = 256;           // threads of the standalone history reduction

@@ -90,7 +90,7 @@ struct PassArgs {
   InstanceStrides is;
   int ninst;         // instances (gridDim.y); 0 or 1: a single field
   const void* sin;   // P4: state read from here and the sum written to out0 (NULL: in place)
-  int nt;            // 1: the step passes store their outputs non-temporally
+  int nt;            // cache hints of the step passes: NT_STORE | NT_LOAD
 };
 
 __device__ __forceinline__ const void* offset_bytes(const void* p, long long q, long long s) {
@@ -157,12 +157,18 @@ constexpr int tw_lds_len() { return tw_in_lds<N, CT>() ? N / 2 : 1; }
 template <int N, class CT, int LEN, int E>
 constexpr bool stab_fits() { return tw_in_lds<N, CT>() && stab_len(LEN, E) * 2 * (int)sizeof(CT) <= 32 * 1024; }
 
-// Store a step pass output.  nt: non-temporal, so the line is not kept in
-// this XCD's L2; the next pass reads it from other XCDs anyway.  Pays while
-// the step's working set stays in the Infinity Cache (the host decides).
+// Cache hints (PassArgs::nt, chosen by the host from the device footprint):
+//   NT_STORE  pass outputs stored non-temporally: not kept in this XCD's L2,
+//             which the next pass (on other XCDs) would not use; pays while
+//             the working set stays in the Infinity Cache;
+//   NT_LOAD   the column passes' line inputs, read once, loaded
+//             non-temporally; pays when the working set lives in HBM, where
+//             it leaves L2 to the row passes' shared transposed lines.
+enum CacheHint { NT_STORE = 1, NT_LOAD = 2 };
+
 template <class T>
 __device__ __forceinline__ void store_out(cplx<T>* p, cplx<T> v, int nt) {
-  if (nt) {
+  if (nt & NT_STORE) {
     typedef T v2 __attribute__((ext_vector_type(2)));
     v2 w;
     w.x = v.x;
@@ -171,6 +177,16 @@ __device__ __forceinline__ void store_out(cplx<T>* p, cplx<T> v, int nt) {
   } else {
     *p = v;
   }
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> load_line(const cplx<T>* p, int nt) {
+  if (nt & NT_LOAD) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 w = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return cplx<T>{w.x, w.y};
+  }
+  return *p;
 }
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
@@ -590,21 +606,21 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     const cplx<CT>* F = static_cast<const cplx<CT>*>(a.in1);
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      v[0][j] = cvt<CT>(S[base + t + TPL * j]);
-      v[1][j] = F[base + t + TPL * j];
+      v[0][j] = cvt<CT>(load_line(&S[base + t + TPL * j], a.nt));
+      v[1][j] = load_line(&F[base + t + TPL * j], a.nt);
     }
   } else {
     const cplx<CT>* X0 = static_cast<const cplx<CT>*>(a.in0);
     const cplx<CT>* X1 = static_cast<const cplx<CT>*>(a.in1);
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      v[0][j] = X0[base + t + TPL * j];
-      v[1][j] = X1[base + t + TPL * j];
+      v[0][j] = load_line(&X0[base + t + TPL * j], a.nt);
+      v[1][j] = load_line(&X1[base + t + TPL * j], a.nt);
     }
     if constexpr (MODE == COL_P4 && Cfg::PF) {
       const cplx<TS>* S = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
 #pragma unroll
-      for (int j = 0; j < E; ++j) sp[j] = S[base + t + TPL * j];
+      for (int j = 0; j < E; ++j) sp[j] = load_line(&S[base + t + TPL * j], a.nt);
     }
   }
   // twiddles after the line loads: a single wait covers both
@@ -737,7 +753,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       if constexpr (!Cfg::PF) {
         const cplx<TS>* Sin = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
 #pragma unroll
-        for (int j = 0; j < E; ++j) sp[j] = Sin[base + t + TPL * j];
+        for (int j = 0; j < E; ++j) sp[j] = load_line(&Sin[base + t + TPL * j], a.nt);
       }
 #pragma unroll
       for (int j = 0; j < E; ++j) {
