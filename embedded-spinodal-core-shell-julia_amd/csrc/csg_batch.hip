@@ -308,7 +308,7 @@ int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const dou
       if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
     }
   }
-  b->nt = nontemporal_for(b->bytes);
+  b->nt = nontemporal_for(b->bytes, true);
   for (BatchClass& k : b->cls) k.nt = b->nt;
   *out = b;
   return CSG_OK;

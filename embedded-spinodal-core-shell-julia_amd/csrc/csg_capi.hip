@@ -311,7 +311,7 @@ int csg_create(csg_ctx** out, int32_t n, double spacing, const csg_params* p, in
     e = hipMemcpy(c->tw, tw32.data(), tw32.size() * sizeof(cplx<float>), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(CSG_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
   }
-  c->nt = nontemporal_for(c->bytes);
+  c->nt = nontemporal_for(c->bytes, false);
   *out = c;
   return CSG_OK;
 }

@@ -64,20 +64,29 @@ std::vector<cplx<T>> make_twiddles(int n) {
   return w;
 }
 
-// Cache hints of the step passes (NT_STORE / NT_LOAD) by device footprint.
-// Non-temporal pass outputs pay while a step's working set stays well inside
-// the 256 MB Infinity Cache and cost beyond it (captured loop: 1024^2 FP64
-// 40 MiB -5 %, mixed 28 MiB -6 %; 2048^2 mixed 113 MiB +6 %, FP64 161 MiB
-// +22 %; 4096^2 and the 256-particle ensemble, 450-950 MiB, +3 to +19 %).
-// Non-temporal column-pass loads are the reverse: 4096^2 mixed -6 %, FP64 and
-// ensembles -1 %, 1024^2 mixed +2 %.  A context or ensemble with a footprint
-// of at most NT_MAX_BYTES streams its stores, a larger one its column loads.
-// CSG_NONTEMPORAL=0..3 (the hint bits) overrides.
+// Cache hints of the step passes (NT_STORE / NT_LOAD / NT_PHI) by device
+// footprint, measured in the captured loop.
+// - Non-temporal pass outputs pay while a step's working set stays well
+//   inside the 256 MB Infinity Cache and cost beyond it: 1024^2 FP64 40 MiB
+//   -5 %, mixed 28 MiB -6 %; 2048^2 mixed 113 MiB +6 %, FP64 161 MiB +22 %;
+//   4096^2 and the 256-particle ensemble, 450-950 MiB, +3 to +19 %.
+// - Non-temporal column-pass loads are the reverse: 4096^2 mixed -6 %, FP64
+//   and ensembles -1 %, 1024^2 mixed +2 %.
+// - Non-temporal phi loads pay for one field far beyond the cache (4096^2
+//   FP64, 641 MiB: -7 %) and cost where part of phi stays cached (4096^2
+//   mixed, 449 MiB: +3 %; ensembles: +1 to +3 %).
+// So: footprint <= NT_MAX_BYTES streams the stores, larger the column loads,
+// and a single field above NT_PHI_MIN_BYTES also phi (a bit that only the FP64
+// row passes of N >= PHI_HINT_MIN_N read).  CSG_NONTEMPORAL=0..7 (the hint
+// bits) overrides.
 constexpr size_t NT_MAX_BYTES = size_t(64) << 20;
-inline int nontemporal_for(size_t bytes) {
+constexpr size_t NT_PHI_MIN_BYTES = size_t(512) << 20;
+inline int nontemporal_for(size_t bytes, bool ensemble) {
   if (const char* e = std::getenv("CSG_NONTEMPORAL"))
-    if (e[0] >= '0' && e[0] <= '3' && e[1] == '\0') return e[0] - '0';
-  return bytes <= NT_MAX_BYTES ? NT_STORE : NT_LOAD;
+    if (e[0] >= '0' && e[0] <= '7' && e[1] == '\0') return e[0] - '0';
+  int h = bytes <= NT_MAX_BYTES ? NT_STORE : NT_LOAD;
+  if (!ensemble && bytes > NT_PHI_MIN_BYTES) h |= NT_PHI;
+  return h;
 }
 
 constexpr int FIN_NT = 256;           // threads of the standalone history reduction

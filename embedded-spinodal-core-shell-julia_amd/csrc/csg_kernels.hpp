@@ -90,7 +90,7 @@ struct PassArgs {
   InstanceStrides is;
   int ninst;         // instances (gridDim.y); 0 or 1: a single field
   const void* sin;   // P4: state read from here and the sum written to out0 (NULL: in place)
-  int nt;            // cache hints of the step passes: NT_STORE | NT_LOAD
+  int nt;            // cache hints of the step passes: NT_STORE | NT_LOAD | NT_PHI
 };
 
 __device__ __forceinline__ const void* offset_bytes(const void* p, long long q, long long s) {
@@ -163,8 +163,12 @@ constexpr bool stab_fits() { return tw_in_lds<N, CT>() && stab_len(LEN, E) * 2 *
 //             the working set stays in the Infinity Cache;
 //   NT_LOAD   the column passes' line inputs, read once, loaded
 //             non-temporally; pays when the working set lives in HBM, where
-//             it leaves L2 to the row passes' shared transposed lines.
-enum CacheHint { NT_STORE = 1, NT_LOAD = 2 };
+//             it leaves L2 to the row passes' shared transposed lines;
+//   NT_PHI    phi loaded non-temporally in the row passes; pays only when a
+//             single field is far larger than the Infinity Cache, and is
+//             honoured by the FP64 row passes of N >= PHI_HINT_MIN_N only.
+enum CacheHint { NT_STORE = 1, NT_LOAD = 2, NT_PHI = 4 };
+constexpr int PHI_HINT_MIN_N = 4096;
 
 template <class T>
 __device__ __forceinline__ void store_out(cplx<T>* p, cplx<T> v, int nt) {
@@ -324,9 +328,24 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   static_assert(B * L == NT * E, "each thread moves E elements of the tile");
   constexpr int G = Cfg::G;
   constexpr bool PF = Cfg::PF != 0 && (MODE == ROW_P1 || MODE == ROW_P3);
+  constexpr bool PHI_HINT = std::is_same<CT, double>::value && N >= PHI_HINT_MIN_N;
   CT ph[E][2];   // phi at this thread's sites (P1, P3)
   auto load_phi = [&]() {
     const CT* phi = static_cast<const CT*>(a.phi);
+    // The NT_PHI branch is compiled only where the hint can be set (FP64
+    // fields of 4096^2 and up); elsewhere it would cost its control flow.
+    if constexpr (PHI_HINT) {
+      if (a.nt & NT_PHI) {
+        typedef CT v2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const v2 w = __builtin_nontemporal_load(reinterpret_cast<const v2*>(phi + rbase + 2 * (t + TPL * j)));
+          ph[j][0] = w.x;
+          ph[j][1] = w.y;
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int k = t + TPL * j;

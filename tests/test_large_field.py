@@ -62,3 +62,19 @@ def test_large_field_invariants_and_mixed_error(pkg, janus):
     # where the spinodal dynamics amplify rounding differences
     assert np.max(np.abs(cm - cf)) < 5e-5
     assert np.max(np.abs(hm[:, 2] - hf[:, 2]) / np.abs(hf[:, 2])) < 1e-6
+
+
+def test_large_field_phi_hint_is_bit_identical(pkg, janus, monkeypatch):
+    """Non-temporal phi loads (on by default for this field) change where phi
+    is cached, never values: 5 steps with and without the hint agree bit for bit."""
+    p, _, c, phi = janus
+    out = []
+    for v in ("2", "6"):
+        say(f"4096^2 FP64, CSG_NONTEMPORAL={v}")
+        monkeypatch.setenv("CSG_NONTEMPORAL", v)
+        s = pkg.Session(N, 1.0, p, "fp64")
+        s.set_fields(c, phi)
+        h = s.run(0.01, 5)
+        out.append((s.field(), h))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

@@ -188,14 +188,14 @@ def test_ensemble_schedules_follow_single_sessions(pkg):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_nontemporal_outputs_are_bit_identical(pkg, monkeypatch, precision):
-    """The cache hints (non-temporal stores and column loads, chosen from the
-    device footprint, forced here with CSG_NONTEMPORAL=0..3) change where data
+    """The cache hints (non-temporal stores, column loads and phi loads, chosen from the
+    device footprint, forced here with CSG_NONTEMPORAL=0..7) change where data
     is cached, never values: fields and histories agree bit for bit, single
     fields and ensembles alike."""
     n, steps = 256, 40
     c, phi = _fields(pkg, n)
     out = {}
-    for v in ("0", "1", "2", "3"):
+    for v in ("0", "1", "2", "3", "4", "7"):
         monkeypatch.setenv("CSG_NONTEMPORAL", v)
         s = pkg.Session(n, precision=precision)
         s.set_fields(c, phi)
@@ -208,6 +208,6 @@ def test_nontemporal_outputs_are_bit_identical(pkg, monkeypatch, precision):
         out[v] = (s.field(), h, ens.field(0), ens.field(1))
         s.close()
         ens.close()
-    for v in ("1", "2", "3"):
+    for v in ("1", "2", "3", "4", "7"):
         for a, b in zip(out["0"], out[v]):
             assert np.array_equal(a, b), v
