@@ -306,8 +306,11 @@ def cpu_baseline(c0, phi, a, c_gpu_steps=None):
 
 def gpu_clocks():
     """The GPU's current clock levels as rocm-smi reports them (None if the
-    tool is not available)."""
+    tool is not available, or under rocprofv3, whose preload would make the
+    child initialise the GPU before rocm-smi's interpreter starts)."""
     import subprocess
+    if any(k.startswith("ROCPROF") for k in os.environ):
+        return None
     try:
         r = subprocess.run(["rocm-smi", "--showclocks"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=30)
