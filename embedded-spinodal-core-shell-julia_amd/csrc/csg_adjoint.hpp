@@ -43,6 +43,18 @@ struct AdjArgs {
 // default otherwise.
 constexpr int adj_E(int len) { return (len % 3 != 0 && len >= 64 && len <= 1024) ? 4 : fft_E(len); }
 
+// Exchange layout of the adjoint FFTs (XM of fft_regs): two buffer sets, one
+// ordering point per stage (ADJ_XM = 2), stepped down to one buffer per array
+// (1) and to one shared buffer (0) while the buffers of `na` arrays over
+// `lines_lp` complex doubles exceed ADJ_XM_LDS bytes.
+constexpr int ADJ_XM = 2;
+constexpr int ADJ_XM_LDS = 64 * 1024;
+constexpr int adj_xm(int na, int lines_lp) {
+  int xm = ADJ_XM;
+  while (xm > 0 && (xm == 1 ? na : 2 * na) * lines_lp * 16 > ADJ_XM_LDS) --xm;
+  return xm;
+}
+
 template <int N>
 struct AdjColCfg {
   static constexpr int E = adj_E(N);
@@ -52,6 +64,10 @@ struct AdjColCfg {
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = (N / 2) / C;
+  // up to three arrays per transform (adj_col3), at stride AS, second set at BS
+  static constexpr int XM = adj_xm(3, C * LP);
+  static constexpr int AS = XM == 0 ? 0 : C * LP, BS = 3 * C * LP;
+  static constexpr int LDS = (XM == 0 ? 1 : XM == 1 ? 3 : 6) * C * LP;
 };
 
 // Stage the values at -ky of `src` through the line buffer (packed line 0).
@@ -81,7 +97,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   using Cfg = AdjColCfg<N>;
   constexpr int E = Cfg::E, TPL = Cfg::TPL, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
-  __shared__ cplx<double> lds[Cfg::C * LP];
+  __shared__ cplx<double> lds[Cfg::LDS];
   __shared__ cplx<double> tw_s[tw_lds_len<N, double>()];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
   const cplx<double>* tw = tw_in_lds<N, double>() ? tw_s : tw_g;
@@ -103,7 +119,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<double, -1, N, 2, N, WAVE, E>(v, t, line, tw);
+  fft_regs<double, -1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, line, tw);
   cplx<double> gn[E], dn[E];
   if (kx == 0) {
     partner_of<N, E, TPL, WAVE>(v[0], gn, t, line);
@@ -136,7 +152,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     v[0][j] = mul_i(sx * V);
     v[1][j] = mul_i(sy * V);
   }
-  fft_regs<double, +1, N, 2, N, WAVE, E>(v, t, line, tw);
+  fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     a.X0[base + t + TPL * j] = v[0][j];
@@ -154,7 +170,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   using Cfg = AdjColCfg<N>;
   constexpr int E = Cfg::E, TPL = Cfg::TPL, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
-  __shared__ cplx<double> lds[Cfg::C * LP];
+  __shared__ cplx<double> lds[Cfg::LDS];
   __shared__ cplx<double> tw_s[tw_lds_len<N, double>()];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
   const cplx<double>* tw = tw_in_lds<N, double>() ? tw_s : tw_g;
@@ -178,7 +194,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
   __syncthreads();
-  fft_regs<double, -1, N, 3, N, WAVE, E>(v, t, line, tw);
+  fft_regs<double, -1, N, 3, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, line, tw);
   cplx<double> jyn[E], cn[E];
   if (kx == 0) {
     partner_of<N, E, TPL, WAVE>(v[1], jyn, t, line);
@@ -213,7 +229,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     w[0][j] = (1.0 / N) * r;
     w[1][j] = (dt / N) * lap;
   }
-  fft_regs<double, +1, N, 2, N, WAVE, E>(w, t, line, tw);
+  fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
     const size_t o = base + t + TPL * j;
@@ -247,6 +263,10 @@ struct AdjRowCfg {
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = N / B;
+  // two arrays per transform; the transposed tiles of both fill the first set
+  static constexpr int XM = adj_xm(2, B * LP);
+  static constexpr int AS = XM == 0 ? 0 : B * LP, BS = 2 * B * LP;
+  static constexpr int LDS = (XM == 2 ? 4 : 2) * B * LP;
 };
 
 template <int N>
@@ -255,7 +275,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   using Cfg = AdjRowCfg<N>;
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
-  __shared__ cplx<double> lds[2 * B * LP];
+  __shared__ cplx<double> lds[Cfg::LDS];
   __shared__ cplx<double> tw_s[tw_lds_len<N, double>()];
   __shared__ double scratch[8 * ((NT + 63) / 64)];
   const cplx<double>* tw = tw_in_lds<N, double>() ? tw_s : tw_g;
@@ -311,7 +331,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     }
   }
   lds_sync<WAVE>();
-  fft_regs<double, +1, L, 2, N, WAVE, E>(v, t, row, tw);
+  fft_regs<double, +1, L, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, row, tw);
   // w = dt f''(c) r_v (core_shell.jl:53-54 differentiated in c); <r_v, g>
   const double A = p.A, Bc = p.B, cb = p.cb, cc = p.cc, dt = a.dt;
   double srg = 0.0;
@@ -333,7 +353,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     }
     w[0][j] = mk(out[0], out[1]);
   }
-  fft_regs<double, -1, L, 1, N, WAVE, E>(w, t, row, tw);
+  fft_regs<double, -1, L, 1, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, row, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) row[pad<double>(t + TPL * j)] = w[0][j];
   lds_sync<WAVE>();
