@@ -11,7 +11,9 @@ kernel schedules on the resident field (--no-tune keeps the defaults).
 
 Also reported:
   roofline      dominant pass of the reported form: algorithmic bytes per
-                launch (per-site bytes x N^2) / its mean HIP-event duration
+                launch (per-site bytes x N^2) / its mean HIP-event duration;
+                measured_copy holds the device copy ceilings (HBM-resident
+                and Infinity-Cache-resident) beside the 8 TB/s spec peak
   cpu_baseline  the FP64 numpy oracle (reference-equivalent: 17 complex
                 N^2 FFTs per iteration), 1 thread, on a bounded sample of the
                 same workload
@@ -86,6 +88,48 @@ def calibrate_copy(mib: int) -> None:
     hip.hipFree(a)
     hip.hipFree(b)
     print(json.dumps({"calibrate_copy_bytes": mib << 20, "repeats": 3}), flush=True)
+
+
+def copy_ceiling(pkg) -> dict:
+    """Measured device copy ceilings (read + write bytes / time, GB/s): the
+    library's streaming copy kernel and the runtime's hipMemcpyAsync, each on
+    a 1 GiB buffer pair, which streams through HBM, and on a 32 MiB pair,
+    which stays inside the 256 MB Infinity Cache like the 1024^2 working set."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def memcpy_gbs(mib, reps):
+        n = ctypes.c_size_t(mib << 20)
+        a, b, e0, e1 = vp(), vp(), vp(), vp()
+        ok(hip.hipMalloc(ctypes.byref(a), n), "hipMalloc")
+        ok(hip.hipMalloc(ctypes.byref(b), n), "hipMalloc")
+        ok(hip.hipMemset(a, 1, n), "hipMemset")
+        ok(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
+        for _ in range(3):
+            ok(hip.hipMemcpyAsync(b, a, n, 3, vp()), "hipMemcpyAsync")   # hipMemcpyDeviceToDevice
+        ok(hip.hipEventRecord(e0, vp()), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(b, a, n, 3, vp()), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, vp()), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+        for e in (e0, e1):
+            hip.hipEventDestroy(e)
+        for q in (a, b):
+            hip.hipFree(q)
+        return 2.0 * (mib << 20) * reps / (ms.value * 1e-3) / 1e9
+
+    return {"copy_kernel": {"hbm_1GiB": pkg.copy_bandwidth(1 << 30, 20),
+                            "cache_32MiB": pkg.copy_bandwidth(32 << 20, 400)},
+            "hipMemcpyAsync": {"hbm_1GiB": memcpy_gbs(1024, 20), "cache_32MiB": memcpy_gbs(32, 400)},
+            "unit": "GB/s"}
 
 
 def _traffic(kernel: str, workload: str):
@@ -368,7 +412,7 @@ def main():
     roof = dict(bound="hbm", achieved=head["dominant_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=head["dominant_gbs"] / HBM_PEAK_GBS, kernel=dom,
                 bytes_per_launch=head["dominant_bytes"], ms_per_launch=head["dominant_ms"],
-                traffic=_traffic(dom, f"{workload}_{pick}"))
+                traffic=_traffic(dom, f"{workload}_{pick}"), measured_copy=copy_ceiling(pkg))
     line = {
         "metric": "site-updates/s at 1024² (FP64 vs mixed, max|Δc|≤1e-5); % HBM roofline",
         "value": head["value"], "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps,

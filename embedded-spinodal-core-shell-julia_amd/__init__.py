@@ -8,7 +8,7 @@ The compute path is ``libcsg.so`` (HIP kernels behind the C ABI of
 adjoint (``adjoint.gradient``) and ensembles (``run_ensemble``).
 """
 from .core_shell import (Grid, NonFiniteCompositionError, Parameters, Session, Shape,  # noqa: F401
-                         bulk_derivative, bulk_energy, chemical_potential, energy, geometry,
+                         bulk_derivative, bulk_energy, chemical_potential, copy_bandwidth, energy, geometry,
                          initial_fields, make_grid, run_simulation, step, transport_rate)
 from ._capi import CsgError, lib  # noqa: F401
 from .writers import julia_float, writedlm  # noqa: F401

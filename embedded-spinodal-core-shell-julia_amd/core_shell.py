@@ -213,6 +213,14 @@ class Session:
         return int(lib().csg_device_bytes(self.handle))
 
 
+def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes over time)
+    of a streaming copy kernel over two buffers of nbytes bytes."""
+    out = ctypes.c_double()
+    check(lib().csg_copy_bandwidth(int(device), int(nbytes), int(reps), ctypes.byref(out)))
+    return out.value
+
+
 def _key(phi: np.ndarray):
     return (phi.shape, hash(phi.tobytes()))
 

@@ -125,6 +125,15 @@ int csg_julia_rand(int64_t seed, int64_t count, double* out);
  * resident field by nsteps steps. */
 int csg_time_passes(csg_ctx* ctx, double dt, int64_t nsteps, double* ms);
 
+/* Measured copy bandwidth of the device: a streaming copy kernel (16-byte
+ * words) over two fresh buffers of `bytes` bytes, 3 untimed and `reps` timed
+ * launches between HIP events; *gbs = 2 * bytes * reps / time, in GB/s (read
+ * plus write).  A diagnostic with no counterpart in the reference: the
+ * measured ceiling that pass bandwidths are read against, beside the
+ * published peak.  bytes: a multiple of 16 in [16, 16 GiB].  Allocates and
+ * frees its buffers; synchronous. */
+int csg_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs);
+
 /* Adjoint of the time-step loop, FP64 contexts only.  No counterpart in the
  * reference (which has no derivatives); it differentiates step
  * (core_shell.jl:72-76) iterated from the resident field.

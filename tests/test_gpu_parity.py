@@ -267,3 +267,15 @@ def test_set_params_matches_fresh_context(pkg):
     assert np.max(np.abs(s.field() - cr)) < 1e-12
     s.close()
     f.close()
+
+
+def test_copy_bandwidth_is_plausible_and_checks_arguments(pkg):
+    """csg_copy_bandwidth: a 64 MiB streaming copy reports a positive rate
+    below a generous bound (read + write, GB/s); sizes that are not a
+    multiple of 16 bytes, or beyond 16 GiB, and zero repeats are rejected."""
+    say("copy bandwidth")
+    gbs = pkg.copy_bandwidth(64 << 20, 10)
+    assert 100.0 < gbs < 50000.0
+    for nbytes, reps in ((24, 1), (0, 1), ((16 << 30) + 16, 1), (1 << 20, 0)):
+        with pytest.raises(pkg.CsgError):
+            pkg.copy_bandwidth(nbytes, reps)
