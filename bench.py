@@ -371,7 +371,7 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no "
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the library has no "
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import __graft_entry__

@@ -1,6 +1,7 @@
 """Oracle (test infrastructure only): CPU adjoint of the reference step.
 
-Only ``tests/`` may import this module, as the checker of the HIP adjoint.
+Only ``tests/`` may import this module, as the reference the HIP adjoint's
+gradients are compared against.
 
 The step of ``core_shell.jl:72-76`` written with real linear operators
 (spectral derivatives D_x, D_y with the Nyquist mode zeroed, D^T = -D;

@@ -1,7 +1,8 @@
 """Oracle (test infrastructure only): FP64 numpy restatement of ``core_shell.jl``.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
-may import this module, and only as the checker or as the timed CPU baseline;
+may import this module, and only as the reference results are compared against
+or as the timed CPU baseline;
 the product path (the HIP library) never calls it.
 
 Arrays are indexed like the Julia matrices: ``a[i, j]`` with ``i`` the first
