@@ -76,10 +76,10 @@ __device__ __forceinline__ void partner_of(const cplx<double> (&src)[E], cplx<do
                                            cplx<double>* line) {
   lds_sync<WAVE>();
 #pragma unroll
-  for (int j = 0; j < E; ++j) line[pad<double>(t + TPL * j)] = src[j];
+  for (int j = 0; j < E; ++j) line[pad<double, N>(t + TPL * j)] = src[j];
   lds_sync<WAVE>();
 #pragma unroll
-  for (int j = 0; j < E; ++j) dst[j] = line[pad<double>(neg_index<N>(t + TPL * j))];
+  for (int j = 0; j < E; ++j) dst[j] = line[pad<double, N>(neg_index<N>(t + TPL * j))];
   lds_sync<WAVE>();
 }
 
@@ -309,8 +309,8 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int f = threadIdx.x + NT * r;
-    lds[(f % B) * LP + pad<double>(f / B)] = tr_x[r];
-    lds[(B + f % B) * LP + pad<double>(f / B)] = tr_s[r];
+    lds[(f % B) * LP + pad<double, L>(f / B)] = tr_x[r];
+    lds[(B + f % B) * LP + pad<double, L>(f / B)] = tr_s[r];
   }
   __syncthreads();
   cplx<double> v[2][E];
@@ -321,11 +321,11 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int k = t + TPL * j;
-      const cplx<double> xk = tr[pad<double>(k)];
+      const cplx<double> xk = tr[pad<double, L>(k)];
       if (k == 0) {
         v[q][j] = mk(scale * (xk.x + xk.y), scale * (xk.x - xk.y));
       } else {
-        const cplx<double> xm = tr[pad<double>(L - k)];
+        const cplx<double> xm = tr[pad<double, L>(L - k)];
         v[q][j] = scale * ((xk + conj(xm)) + mul_i(cmul(xk - conj(xm), conj(tw[k]))));
       }
     }
@@ -355,7 +355,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   fft_regs<double, -1, L, 1, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, row, tw);
 #pragma unroll
-  for (int j = 0; j < E; ++j) row[pad<double>(t + TPL * j)] = w[0][j];
+  for (int j = 0; j < E; ++j) row[pad<double, L>(t + TPL * j)] = w[0][j];
   lds_sync<WAVE>();
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -364,19 +364,19 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     if (k == 0) {
       w[0][j] = mk(zk.x + zk.y, zk.x - zk.y);
     } else {
-      const cplx<double> zm = row[pad<double>(L - k)];
+      const cplx<double> zm = row[pad<double, L>(L - k)];
       w[0][j] = 0.5 * (zk + conj(zm)) + cmul(tw[k], 0.5 * mul_mi(zk - conj(zm)));
     }
   }
   lds_sync<WAVE>();
 #pragma unroll
-  for (int j = 0; j < E; ++j) row[pad<double>(t + TPL * j)] = w[0][j];
+  for (int j = 0; j < E; ++j) row[pad<double, L>(t + TPL * j)] = w[0][j];
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int f = threadIdx.x + NT * r;
     const int bb = f % B, kx = f / B;
-    a.G[(size_t)kx * N + j0 + bb] = gp[r] + lds[bb * LP + pad<double>(kx)];
+    a.G[(size_t)kx * N + j0 + bb] = gp[r] + lds[bb * LP + pad<double, L>(kx)];
   }
   double rr[1] = {srg};
   const int ops[1] = {RED_SUM};

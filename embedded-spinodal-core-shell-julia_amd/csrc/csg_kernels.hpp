@@ -392,7 +392,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 #pragma unroll
         for (int r = 0; r < G; ++r) {
           const int f = threadIdx.x + NT * (g0 + r);
-          lds[(q * B + f % B) * LP + pad<CT>(f / B)] = cvt<CT>(tl[q][r]);
+          lds[(q * B + f % B) * LP + pad<CT, L>(f / B)] = cvt<CT>(tl[q][r]);
         }
     }
     __syncthreads();
@@ -405,11 +405,11 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 #pragma unroll
       for (int j = 0; j < E; ++j) {
         const int k = t + TPL * j;
-        const cplx<CT> xk = tr[pad<CT>(k)];
+        const cplx<CT> xk = tr[pad<CT, L>(k)];
         if (k == 0) {
           v[q][j] = mk(scale * (xk.x + xk.y), scale * (xk.x - xk.y));
         } else {
-          const cplx<CT> xm = tr[pad<CT>(L - k)];
+          const cplx<CT> xm = tr[pad<CT, L>(L - k)];
           v[q][j] = scale * ((xk + conj(xm)) + mul_i(cmul(xk - conj(xm), conj(tw[k]))));
         }
       }
@@ -493,7 +493,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   for (int q = 0; q < NARR; ++q) {
     cplx<CT>* tr = lds + (q * B + b) * LP;
 #pragma unroll
-    for (int j = 0; j < E; ++j) tr[pad<CT>(t + TPL * j)] = v[q][j];
+    for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
   }
   lds_sync<WAVE>();
   const CT half = static_cast<CT>(0.5);
@@ -507,7 +507,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       if (k == 0) {
         v[q][j] = mk(zk.x + zk.y, zk.x - zk.y);
       } else {
-        const cplx<CT> zm = tr[pad<CT>(L - k)];
+        const cplx<CT> zm = tr[pad<CT, L>(L - k)];
         const cplx<CT> ek = half * (zk + conj(zm));
         const cplx<CT> ok = half * mul_mi(zk - conj(zm));
         v[q][j] = ek + cmul(tw[k], ok);
@@ -519,7 +519,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   for (int q = 0; q < NARR; ++q) {
     cplx<CT>* tr = lds + (q * B + b) * LP;
 #pragma unroll
-    for (int j = 0; j < E; ++j) tr[pad<CT>(t + TPL * j)] = v[q][j];
+    for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
   }
   __syncthreads();
 #pragma unroll
@@ -529,7 +529,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     for (int r = 0; r < E; ++r) {
       const int f = threadIdx.x + NT * r;
       const int bb = f % B, kx = f / B;
-      store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT>(kx)]), a.nt);
+      store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT, L>(kx)]), a.nt);
     }
   }
 
@@ -655,10 +655,10 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     constexpr int PA = (MODE == COL_P2 || MODE == COL_MU) ? 0 : 1;
     lds_sync<WAVE>();
 #pragma unroll
-    for (int j = 0; j < E; ++j) line[pad<CT>(t + TPL * j)] = v[PA][j];
+    for (int j = 0; j < E; ++j) line[pad<CT, N>(t + TPL * j)] = v[PA][j];
     lds_sync<WAVE>();
 #pragma unroll
-    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT>(neg_index<N>(t + TPL * j))];
+    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT, N>(neg_index<N>(t + TPL * j))];
     lds_sync<WAVE>();
   }
 

@@ -32,9 +32,9 @@ void run_stages(std::vector<std::vector<cplx<T>>>& v, const cplx<T>* tw, std::ve
       constexpr int NBT = E / R;
       for (int t = 0; t < TPL; ++t)
         for (int m = 0; m < NBT; ++m)
-          for (int q = 0; q < R; ++q) line[pad<T>(stockham_out(t + TPL * m, q, R, P))] = v[t][m + q * NBT];
+          for (int q = 0; q < R; ++q) line[pad<T, LEN>(stockham_out(t + TPL * m, q, R, P))] = v[t][m + q * NBT];
       for (int t = 0; t < TPL; ++t)
-        for (int j = 0; j < E; ++j) v[t][j] = line[pad<T>(t + TPL * j)];
+        for (int j = 0; j < E; ++j) v[t][j] = line[pad<T, LEN>(t + TPL * j)];
     }
     run_stages<DIR, LEN, S + 1, T, NTAB, E, ST>(v, tw, line);
   }
