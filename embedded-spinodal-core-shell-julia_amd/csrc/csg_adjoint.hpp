@@ -259,7 +259,7 @@ struct AdjRowCfg {
   static constexpr int E = adj_E(L);
   static constexpr int TPL = L / E;
   static constexpr int B = adj_rows<N, E>();
-  static constexpr int LP = padded_len<double>(L);
+  static constexpr int LP = tile_stride<double>(L, B);
   static constexpr int NT = B * TPL;
   static constexpr bool WAVE = TPL <= 64;
   static constexpr int GRID = N / B;

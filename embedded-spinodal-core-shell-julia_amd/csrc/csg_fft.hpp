@@ -202,21 +202,41 @@ constexpr int plan_pstride(int len, int s, bool inv, int e) {
 
 // LDS slot of element e of a line buffer of LEN elements (buffers hold
 // padded_len<T>(LEN) slots).  Lines of up to SWZ_MAX_LEN elements: an XOR
-// swizzle inside aligned blocks of 8 (16-byte elements) or 16 (8-byte
-// elements); longer lines: one pad slot per 128 bytes of line data.  Chosen
-// from bank-conflict counts of ds_write/ds_read b128 and b64 on MI355X over
-// the exchange patterns: the swizzle removes the conflicts of the radix-4
-// stages' writes and of the consecutive reads (1024^2 FP64: 61 % fewer
-// conflict cycles), the pad slots suit the radix-8/16 stages of long lines.
+// swizzle that permutes aligned blocks of 8 (16-byte elements: low 3 bits
+// ^= bits 3..5) or 32 (8-byte elements: bits 0..4 ^= a mix of bits 4..7);
+// longer lines: one pad slot per 128 bytes of line data.  Chosen from
+// bank-conflict counts of ds_write/ds_read b128 and b64 on MI355X over the
+// exchange patterns of the radix-4/8 stages and the consecutive reads, then
+// by the step time (1024^2 FP64: 61 % fewer conflict cycles); the pad slots
+// suit the radix-8/16 stages of long lines.
 constexpr int SWZ_MAX_LEN = 1024;
 template <class T> constexpr int pad_shift() { return sizeof(T) == 8 ? 3 : 4; }
 template <class T, int LEN> CSG_HD int pad(int e) {
-  if constexpr (LEN <= SWZ_MAX_LEN)
-    return sizeof(T) == 8 ? (e ^ ((e >> 3) & 7)) : (e ^ ((e >> 4) & 15));
-  else
+  if constexpr (LEN <= SWZ_MAX_LEN) {
+    if constexpr (sizeof(T) == 8)
+      return e ^ ((e >> 3) & 7);
+    else
+      return e ^ (((e >> 4) & 1) * 5) ^ (((e >> 5) & 1) * 10) ^ (((e >> 6) & 1) * 12) ^ (((e >> 7) & 1) * 16);
+  } else {
     return e + (e >> pad_shift<T>());
+  }
 }
 template <class T> constexpr int padded_len(int len) { return len + (len >> pad_shift<T>()) + 1; }
+
+// Row stride (slots) of a tile of b rows of len elements whose transposed
+// loads and stores put the b rows of one element in neighbouring lanes.  For
+// 8-byte elements the rows' base slots are spread over the banks, stride =
+// 16/b modulo 16, the conflict-free strides measured for b = 2..8 (1024^2
+// mixed step -2.4 %).  16-byte elements keep padded_len: their conflict-free
+// strides (8/b modulo 16) made the 1024^2 FP64 step no faster.  Rows longer
+// than SWZ_MAX_LEN keep padded_len.
+template <class T>
+constexpr int tile_stride(int len, int b) {
+  const int base = padded_len<T>(len);
+  const int per = 16 / (b > 0 ? b : 1);
+  if (sizeof(T) == 8 || len > SWZ_MAX_LEN || b < 2 || per < 1) return base;
+  return base + ((per - base % 16) % 16 + 16) % 16;
+}
 
 // Index of -k on a periodic line of length N.
 template <int N> CSG_HD int neg_index(int k) { return k == 0 ? 0 : N - k; }

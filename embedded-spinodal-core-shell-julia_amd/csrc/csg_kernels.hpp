@@ -284,7 +284,7 @@ struct RowCfg {
   static constexpr int G = SC::G ? SC::G : E;
   static constexpr int PF = SC::PF;
   static constexpr int XM = SC::X >= 0 ? SC::X : 0;
-  static constexpr int LP = padded_len<CT>(L);
+  static constexpr int LP = tile_stride<CT>(L, B);   // row stride of the tile
   static constexpr int AS = B * LP, BS = NARR * B * LP;   // exchange strides
   static constexpr bool ST = SC::T == 1;
   static_assert(!ST || stab_fits<N, CT, L, E>(), "stage tables do not fit this schedule");
