@@ -12,7 +12,8 @@
 // sweep over those, and the list is ordered by that sweep's times (in the
 // comments).  For the ensemble sizes, the fastest (E, W, G, PF, X, T) of the
 // second sweep run over 64 instances per launch is appended when it is not
-// already listed.  The first entry is the default; the runtime tuner
+// already listed.  Where the captured loop ranks them differently (marked),
+// the loop's order wins.  The first entry is the default; the runtime tuner
 // (csg_autotune, csg_batch_autotune) times the candidates inside the captured
 // step loop and keeps the fastest combination.  Sizes without an entry use the
 // single rule-based schedule Sched<>.
@@ -77,7 +78,10 @@ template <> struct StepCands<768, false, 1> { using type = SchedList<Sched<6, 4,
 template <> struct StepCands<768, false, 2> { using type = SchedList<Sched<12, 2, 0, 1, 0, 0>, Sched<12, 1, 0, 1, 0, 0>, Sched<6, 1, 0, 1, 2, 0>>; };  // 10.4 / 10.8 / 11.5 us
 template <> struct StepCands<768, false, 3> { using type = SchedList<Sched<6, 8, 1, 0, 2, 1>, Sched<12, 8, 1, 0, 2, 0>, Sched<6, 2, 1, 0, 2, 1>>; };  // 9.5 / 9.6 / 9.8 us
 template <> struct StepCands<768, false, 4> { using type = SchedList<Sched<12, 2, 0, 0, 0, 0>, Sched<12, 1, 0, 0, 0, 0>, Sched<6, 1, 0, 0, 2, 0>>; };  // 9.5 / 9.7 / 9.8 us
-template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 2, 0>, Sched<4, 2, 1, 0, 2, 1>>; };  // 11.4 / 12.8 / 12.9 us
+// 1024^2 FP64 P1: ranked by the captured loop's step time, where two rows
+// of two waves each beat four one-wave rows (50.2 vs 50.8 us per step)
+// although they lose alone (isolated 11.2 us for Sched<8, 4, ...>)
+template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 2, 0>>; };
 template <> struct StepCands<1024, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<16, 2, 0, 1, 0, 0>>; };  // 13.1 / 14.1 / 14.8 us
 template <> struct StepCands<1024, true, 3> { using type = SchedList<Sched<8, 8, 1, 0, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 0, 0>>; };  // 14.6 / 14.9 / 16.6 us
 template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // 10.4 / 11.1 / 12.4 us
