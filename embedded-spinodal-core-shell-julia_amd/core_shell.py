@@ -127,7 +127,7 @@ class Session:
         self._fin = weakref.finalize(self, lib().csg_destroy, h)
         if schedule == "cached":
             from . import schedules
-            schedule = schedules.load(self.n, precision)
+            schedule = schedules.load(self.n, precision, self.schedule_info())
         if schedule is not None:
             self.set_schedule(schedule)
 
@@ -332,7 +332,7 @@ def run_simulation(c0, phi, dt: float = 0.01, steps: int = 1000, spacing: float 
     s = _session(c0.shape[0], spacing, p, precision, device)
     if schedule == "cached":
         from . import schedules
-        schedule = schedules.load(s.n, precision)
+        schedule = schedules.load(s.n, precision, s.schedule_info())
     s.set_schedule((0, 0, 0, 0) if schedule is None else schedule)
     s.set_fields(c0, phi)
     hist = s.run(dt, steps, history_every)

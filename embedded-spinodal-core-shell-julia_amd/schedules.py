@@ -1,7 +1,10 @@
 """Cache of tuned kernel schedules (``Session.autotune`` results).
 
-One JSON file maps "N|precision" to the selected candidate per pass and the
-measured device time per loop iteration.  Location: ``$CSG_SCHEDULE_CACHE``,
+One JSON file maps "N|precision" to the selected candidate per pass, the
+measured device time per loop iteration and, when the caller supplies the
+candidate table (``Session.schedule_info``), the resolved schedule of each
+chosen candidate: a library rebuilt with a different candidate table then
+no longer matches, and the entry is ignored.  Location: ``$CSG_SCHEDULE_CACHE``,
 else ``~/.cache/csg_amd/schedules.json``.  Nothing is read unless a caller asks
 for ``schedule="cached"``: the default schedules are fixed, so results do not
 depend on what a previous tuning run measured.
@@ -26,15 +29,27 @@ def _read() -> dict:
         return {}
 
 
-def load(n: int, precision: str) -> Optional[tuple]:
-    """The cached selection for (n, precision), or None."""
+def load(n: int, precision: str, info: Optional[list] = None) -> Optional[tuple]:
+    """The cached selection for (n, precision), or None.  With ``info`` (the
+    library's candidate table), an entry that recorded its schedules is
+    returned only if the same indices still select the same schedules."""
     e = _read().get(f"{int(n)}|{precision}")
-    return tuple(e["sel"]) if e else None
+    if not e:
+        return None
+    sel = tuple(int(x) for x in e["sel"])
+    if info is not None and "sched" in e:
+        for q in range(4):
+            if not (0 <= sel[q] < len(info[q])) or dict(info[q][sel[q]]) != e["sched"][q]:
+                return None
+    return sel
 
 
-def save(n: int, precision: str, sel, us_per_step: float) -> None:
+def save(n: int, precision: str, sel, us_per_step: float, info: Optional[list] = None) -> None:
     d = _read()
-    d[f"{int(n)}|{precision}"] = {"sel": [int(x) for x in sel], "us_per_step": float(us_per_step)}
+    entry = {"sel": [int(x) for x in sel], "us_per_step": float(us_per_step)}
+    if info is not None:
+        entry["sched"] = [dict(info[q][int(sel[q])]) for q in range(4)]
+    d[f"{int(n)}|{precision}"] = entry
     p = path()
     os.makedirs(os.path.dirname(p), exist_ok=True)
     tmp = p + ".tmp"
@@ -53,8 +68,9 @@ def tune(n: int, precision: str = "fp64", dt: float = 0.01, rounds: int = 2, *, 
     try:
         s.set_fields(c0, phi)
         sel, us = s.autotune(dt, rounds)
+        info = s.schedule_info()
     finally:
         s.close()
     if store:
-        save(n, precision, sel, us)
+        save(n, precision, sel, us, info)
     return sel, us

@@ -32,6 +32,20 @@ def test_schedule_cache_roundtrip(pkg, tmp_path, monkeypatch):
     assert pkg.schedules.load(512, "fp64") is None
 
 
+def test_schedule_cache_rejects_a_changed_candidate_table(pkg, tmp_path, monkeypatch):
+    """An entry saved with the candidate table is ignored once the same
+    indices select different schedules (a rebuilt library)."""
+    monkeypatch.setenv("CSG_SCHEDULE_CACHE", str(tmp_path / "s.json"))
+    a = {"E": 8, "W": 4, "G": 1, "PF": 0, "X": 0, "T": 1}
+    b = {"E": 4, "W": 2, "G": 1, "PF": 0, "X": 2, "T": 1}
+    info = [[a, b], [a], [a], [a]]
+    pkg.schedules.save(1024, "fp64", (1, 0, 0, 0), 50.0, info)
+    assert pkg.schedules.load(1024, "fp64", info) == (1, 0, 0, 0)
+    assert pkg.schedules.load(1024, "fp64", [[b, a], [a], [a], [a]]) is None   # reordered table
+    assert pkg.schedules.load(1024, "fp64", [[a], [a], [a], [a]]) is None      # index out of range
+    assert pkg.schedules.load(1024, "fp64") == (1, 0, 0, 0)                     # no table: trusted
+
+
 # ---- GPU -------------------------------------------------------------------
 
 @pytest.mark.gpu
