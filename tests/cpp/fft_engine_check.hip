@@ -135,9 +135,31 @@ int check_layout(int n) {
   return 0;
 }
 
+// The LDS slot map of a line buffer of LEN elements is injective into the
+// buffer's padded_len slots, and a tile's row stride leaves every row its
+// padded_len slots, for 16- and 8-byte elements and b = 1..16 rows.
+template <class T, int LEN>
+int check_slots() {
+  std::vector<char> used(padded_len<T>(LEN), 0);
+  for (int e = 0; e < LEN; ++e) {
+    const int s = pad<T, LEN>(e);
+    if (s < 0 || s >= padded_len<T>(LEN) || used[s]) { printf("LEN=%d %s: slot map not injective at %d\n", LEN, sizeof(T) == 8 ? "f64" : "f32", e); return 1; }
+    used[s] = 1;
+  }
+  for (int b = 1; b <= 16; b *= 2)
+    if (tile_stride<T>(LEN, b) < padded_len<T>(LEN)) { printf("LEN=%d b=%d: tile stride too short\n", LEN, b); return 1; }
+  return 0;
+}
+template <int LEN>
+int check_slots_both() { return check_slots<double, LEN>() + check_slots<float, LEN>(); }
+
 int main() {
   int bad = 0;
   for (int n : {16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 48, 96, 192, 384, 768}) bad += check_layout(n);
+  bad += check_slots_both<8>() + check_slots_both<16>() + check_slots_both<24>() + check_slots_both<32>() +
+         check_slots_both<48>() + check_slots_both<64>() + check_slots_both<96>() + check_slots_both<128>() +
+         check_slots_both<192>() + check_slots_both<256>() + check_slots_both<384>() + check_slots_both<512>() +
+         check_slots_both<768>() + check_slots_both<1024>() + check_slots_both<2048>() + check_slots_both<4096>();
   bad += check<8, double>(1e-14);
   bad += check<16, double>(1e-14);
   bad += check<32, double>(1e-14);

@@ -110,10 +110,11 @@ def test_writedlm_layout(pkg, tmp_path):
 
 def test_fft_engine_host_check(tmp_path):
     """The register Stockham engine's stage code (plans, radix-16/8/4/2
-    butterflies, padded exchanges, twiddles from the half table and from the
-    per-stage tables) against a long-double DFT for every line length
-    8..4096, FP64 and FP32, and the twiddle-buffer layout -- compiled for the
-    host."""
+    butterflies, swizzled or padded exchanges, twiddles from the half table
+    and from the per-stage tables) against a long-double DFT for every line
+    length 8..4096, FP64 and FP32; the twiddle-buffer layout; and the LDS
+    slot map (injective into the buffer) and tile row strides -- compiled for
+    the host."""
     import subprocess
     exe = tmp_path / "fft_engine_check"
     src = os.path.join(ROOT, "tests", "cpp", "fft_engine_check.hip")
