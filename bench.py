@@ -243,6 +243,7 @@ def large_field(pkg, a):
         ms = r["ms"]
         out[form] = {"site_updates_per_s": sites * (a.large_steps + 1) / r["secs"],
                      "ms_per_step": 1e3 * r["secs"] / (a.large_steps + 1),
+                     "steps_per_s": (a.large_steps + 1) / r["secs"],
                      "pass_gbs": {PASS_NAMES[k]: PASS_BYTES[form][k] * sites / (ms[k] * 1e-3) / 1e9 for k in range(4)},
                      "pass_ms": {PASS_NAMES[k]: float(ms[k]) for k in range(4)},
                      "step_frac_hbm": sum(PASS_BYTES[form]) * sites * (a.large_steps + 1) / r["secs"] / 1e9 / HBM_PEAK_GBS,
@@ -293,6 +294,7 @@ def ensemble_rate(pkg, a):
         byts = sites * sum(PASS_BYTES[form]) * iters   # packed line layout: no N/2+1 column
         gbs = byts / r["secs"] / 1e9
         out[form] = {"site_updates_per_s": sites * iters / r["secs"], "ms_per_step": 1e3 * r["secs"] / iters,
+                     "steps_per_s": iters / r["secs"],
                      "algorithmic_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "device_bytes": r["bytes"],
                      "autotuned_schedules": r["tuned"]}
     dc = [float(np.max(np.abs(x - y))) for x, y in zip(res["mixed"]["c"], res["fp64"]["c"])]
@@ -389,7 +391,7 @@ def main():
         per = [PASS_BYTES[f][k] * sites / (ms[k] * 1e-3) / 1e9 for k in range(4)]
         dom = int(np.argmax(ms[:4]))
         step_bytes = sum(PASS_BYTES[f]) * sites
-        o = dict(value=ups, ms_per_step=1e3 * r["seconds"] / (a.steps + 1),
+        o = dict(value=ups, ms_per_step=1e3 * r["seconds"] / (a.steps + 1), steps_per_s=(a.steps + 1) / r["seconds"],
                  pass_ms=dict(zip(PASS_NAMES + ("finalize",), [float(x) for x in ms])),
                  pass_gbs=dict(zip(PASS_NAMES, per)),
                  step_gbs=step_bytes / (sum(ms) * 1e-3) / 1e9,
