@@ -4,6 +4,7 @@ oracle bit for bit; argument validation; the Julia CSV number format."""
 import ctypes
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -142,3 +143,13 @@ def test_adjoint_default_checkpointing(pkg):
     fit = budget // (n * n * 8) - 1          # largest step count whose states fit
     assert pkg.adjoint.default_ckpt_every(n, fit) == 1
     assert pkg.adjoint.default_ckpt_every(n, fit + 1) == max(1, round((fit + 1) ** 0.5))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says so,
+    before any GPU work."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 2 and r.stdout == ""
+    assert "one GPU only" in r.stderr
