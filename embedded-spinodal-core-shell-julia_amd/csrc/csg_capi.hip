@@ -751,6 +751,7 @@ int csg_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs)
     (void)hipGetLastError();
     return fail(CSG_ERR_HIP, std::string("csg_copy_bandwidth: ") + hipGetErrorString(e));
   }
+  if (!(ms > 0.0f)) return fail(CSG_ERR_HIP, "csg_copy_bandwidth: the timed copies measured no elapsed time");
   *gbs = 2.0 * static_cast<double>(nb) * reps / (static_cast<double>(ms) * 1e-3) / 1e9;
   return CSG_OK;
 }
