@@ -2,21 +2,25 @@
 
 Workload (one "step" = one iteration of run_simulation's loop, core_shell.jl:
 90-95, history row included): N = 1024, R = 320, w = 2, dt = 0.01, seed 5749,
-default Parameters, fields resident in HBM before the timed region.  The FP64
-form and the mixed form (FP32 FFTs/pointwise, FP64 state and reductions) run
-side by side on the same inputs; the mixed form is reported as `value` when
-its max|dc| against the FP64 GPU run stays <= 1e-5, else the FP64 form.
-Before each timed run the library's runtime tuner (csg_autotune) picks the
-kernel schedules on the resident field (--no-tune keeps the defaults).
+default Parameters, fields resident in HBM before the timed region.  `value`
+is the FP64 form, the reference's precision.  The mixed form (FP32 FFTs and
+pointwise work; FP64 state, k = 0 mode, mean and reductions) runs beside it on
+the same inputs and is reported with its speed-up and its error against the
+FP64 GPU run at a fixed 1000-step horizon (independent of --steps).  Before
+each timed run the library's runtime tuner (csg_autotune) picks the kernel
+schedules on the resident field (--no-tune keeps the defaults).
 
 Also reported:
-  roofline      dominant pass of the reported form: algorithmic bytes per
-                launch (per-site bytes x N^2) / its mean HIP-event duration;
-                measured_copy holds the device copy ceilings (HBM-resident
-                and Infinity-Cache-resident) beside the 8 TB/s spec peak
+  roofline      dominant pass of the FP64 form: algorithmic bytes per launch
+                (per-site bytes x N^2) / its mean in-loop HIP-event duration
+                (timing "hip_event_in_loop"); beside it the rocprofv3
+                kernel-trace average and HBM traffic committed under
+                profiles/ for the same pass, form and schedule (null when the
+                schedule run here has no committed profile); measured_copy
+                holds the device copy ceilings beside the 8 TB/s spec peak
   cpu_baseline  the FP64 numpy oracle (reference-equivalent: 17 complex
                 N^2 FFTs per iteration), 1 thread, on a bounded sample of the
-                same workload
+                same workload, with the host's CPU model and thread counts
   ensemble      the uneven 256-particle ensemble (mixed and FP64 forms)
   large_field   4096^2 Janus-type field, 5000 steps, per-pass HBM GB/s, mixed error
   gradient      gradient evaluations/s of the 512^2 adjoint workload
@@ -132,14 +136,66 @@ def copy_ceiling(pkg) -> dict:
             "unit": "GB/s"}
 
 
-def _traffic(kernel: str, workload: str):
+def _sched_str(sched) -> str:
+    return ", ".join(str(int(x)) for x in sched)
+
+
+def _traffic(kernel: str, workload: str, sched: str):
     """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
-    if a summary for this kernel and workload has been committed."""
+    if a summary for this kernel, workload and schedule has been committed."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            d = json.load(f)
+        if d.get("schedule") == sched:
+            return d.get("bytes_per_launch")
     return None
+
+
+def _rocprof_us(kernel: str, workload: str, form: str, sched: str):
+    """rocprofv3 kernel-trace average duration (us) of this pass, form and
+    schedule from the committed summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"kernel_trace_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            for e in json.load(f).get("kernels", []):
+                if e.get("pass") == kernel and e.get("form") == form and e.get("schedule") == sched:
+                    return e.get("avg_us")
+    return None
+
+
+def _device_sync():
+    """hipDeviceSynchronize: the barrier on both sides of a timed region."""
+    import ctypes
+    if ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def host_cpu() -> dict:
+    """The host's CPU as this process sees it: usable threads, model name and
+    threads per core (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "model": None, "threads_per_core": None}
+    try:
+        r = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=20)
+        for ln in r.stdout.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Thread(s) per core":
+                info["threads_per_core"] = int(v.strip())
+    except (OSError, ValueError, subprocess.TimeoutExpired):
+        pass
+    if info["model"] is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for ln in f:
+                    if ln.startswith("model name"):
+                        info["model"] = ln.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
 
 
 def run_form(pkg, form, c0, phi, a):
@@ -153,21 +209,27 @@ def run_form(pkg, form, c0, phi, a):
     if a.warmup:
         s.run(a.dt, a.warmup)                 # also captures the step graph
     s.set_fields(c0, phi)
+    _device_sync()
     t0 = time.perf_counter()
     hist = s.run(a.dt, a.steps)               # synchronous: returns after the stream drains
+    _device_sync()
     t1 = time.perf_counter()
     c = s.field()
-    # five shorter repeats (continuing from the field) for a median and spread
+    # five repeats of at least 200 steps (continuing from the field) for a
+    # median and spread
     reps = []
+    k = max(200, a.steps // 10)
     for _ in range(5):
-        k = max(1, a.steps // 10)
         r0 = time.perf_counter()
         s.run(a.dt, k)
         reps.append(1e3 * (time.perf_counter() - r0) / (k + 1))
     ms_pass = s.time_passes(a.dt, 50)
     sched = list(s.schedule)
+    info = s.schedule_info()
+    resolved = [_sched_str(info[q][sched[q]][k] for k in ("E", "W", "G", "PF", "X", "T")) for q in range(4)]
     s.close()
-    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, tune=tune, reps=reps)
+    return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, resolved=resolved, tune=tune,
+                reps=reps, rep_steps=k)
 
 
 def gradient_rate(pkg, a):
@@ -237,7 +299,9 @@ def large_field(pkg, a):
         res[form] = dict(secs=secs, hist=hist, c=s.field(), ms=s.time_passes(a.dt, 20), sched=s.schedule)
         s.close()
     out = {"workload": f"large field {n}x{n}, R=1300, w=2, Janus-type set B={p.B:g}, chi={p.chi:g}, Q={p.Q:g}, "
-                       f"{a.large_steps} steps, dt={a.dt:g}, history every step"}
+                       f"{a.large_steps} steps, dt={a.dt:g}, history every step",
+           "deviation": "Q is doubled as well (the stated set changes only B and chi): with Q=0.5 the "
+                        "stabilised scheme turns non-finite at dt=0.01 (step 964 at 4096^2) in both forms"}
     sites = n * n
     for form, r in res.items():
         ms = r["ms"]
@@ -307,17 +371,22 @@ def ensemble_rate(pkg, a):
     return out
 
 
-def short_horizon_error(pkg, form, c0, phi, a, steps=1000):
-    """max|dc| of `form` against the FP64 GPU run after `steps` steps (t = 10 at
-    dt = 0.01, the horizon of the FP64 parity gate)."""
+def horizon_errors(pkg, form, c0, phi, a, steps=1000):
+    """`form` against the FP64 GPU run on the same inputs after `steps` steps
+    (t = 10 at dt = 0.01, the horizon of the FP64 parity gate), whatever
+    --steps is: max|dc|, the largest relative energy error over the history,
+    and the form's mass drift."""
     out = []
     for f in ("fp64", form):
         s = pkg.Session(a.n, 1.0, pkg.Parameters(), f)
         s.set_fields(c0, phi)
-        s.run(a.dt, steps)
-        out.append(s.field())
+        h = s.run(a.dt, steps)
+        out.append((s.field(), h))
         s.close()
-    return float(np.max(np.abs(out[1] - out[0])))
+    (c64, h64), (cf, hf) = out
+    return {"steps": steps, "max_dc_vs_fp64": float(np.max(np.abs(cf - c64))),
+            "rel_energy_err_vs_fp64": float(np.max(np.abs(hf[:, 2] - h64[:, 2]) / np.abs(h64[:, 2]))),
+            "mass_drift": float(np.max(np.abs(hf[:, 1] - hf[0, 1])))}
 
 
 def cpu_threads_baseline(c0, phi, a, threads):
@@ -331,8 +400,9 @@ def cpu_threads_baseline(c0, phi, a, threads):
     finally:
         orc.set_fft_workers(0)
     return dict(value=a.n * a.n * (a.cpu_steps + 1) / secs, unit="site-updates/s", cores=threads, kind="port",
-                sample=f"oracle run_simulation with 2D FFTs on {threads} threads (scipy pocketfft), "
-                       f"{a.cpu_steps + 1} loop iterations of the {a.n}^2 workload in {secs:.1f} s")
+                sample=f"oracle run_simulation with its 2D FFTs on {threads} threads (scipy pocketfft) and "
+                       f"its elementwise numpy work on 1 thread, {a.cpu_steps + 1} loop iterations of the "
+                       f"{a.n}^2 workload in {secs:.1f} s")
 
 
 def cpu_baseline(c0, phi, a, c_gpu_steps=None):
@@ -345,7 +415,7 @@ def cpu_baseline(c0, phi, a, c_gpu_steps=None):
     secs = t1 - t0
     # run_simulation evaluates steps+1 loop iterations (the last without advancing)
     rate = a.n * a.n * (a.cpu_steps + 1) / secs
-    return c, dict(value=rate, unit="site-updates/s", cores=1, kind="port",
+    return c, dict(value=rate, unit="site-updates/s", cores=1, kind="port", host=host_cpu(),
                    sample=f"oracle run_simulation (numpy FP64, 17 complex {a.n}^2 FFTs per iteration), "
                           f"{a.cpu_steps + 1} loop iterations of the {a.n}^2 workload in {secs:.1f} s")
 
@@ -380,6 +450,7 @@ def main():
     pkg = __graft_entry__.load_package()
     c0, phi = pkg.initial_fields(a.n, radius=a.radius, width=a.width, seed=5749)
     forms = [f for f in a.forms.split(",") if f]
+    forms = sorted(forms, key=lambda f: f != "fp64")          # FP64 first
     res = {f: run_form(pkg, f, c0, phi, a) for f in forms}
     sites = a.n * a.n
     workload = f"{a.n}x{a.n}"
@@ -396,25 +467,31 @@ def main():
                  pass_gbs=dict(zip(PASS_NAMES, per)),
                  step_gbs=step_bytes / (sum(ms) * 1e-3) / 1e9,
                  dominant=PASS_NAMES[dom], dominant_gbs=per[dom], dominant_ms=float(ms[dom]),
-                 dominant_bytes=PASS_BYTES[f][dom] * sites,
+                 dominant_bytes=PASS_BYTES[f][dom] * sites, dominant_schedule=r["resolved"][dom],
                  energy_final=float(r["hist"][-1, 2]),
                  mass_drift=float(np.max(np.abs(r["hist"][:, 1] - np.mean(c0)))),
-                 schedule=r["schedule"], autotune=r["tune"],
-                 ms_per_step_repeats=[float(x) for x in r["reps"]],
+                 schedule=r["schedule"], schedule_resolved=r["resolved"], autotune=r["tune"],
+                 ms_per_step_repeats=[float(x) for x in r["reps"]], steps_per_repeat=r["rep_steps"],
                  ms_per_step_median_of_repeats=float(np.median(r["reps"])))
         if ref is not None and f != "fp64":
-            o["max_dc_vs_fp64"] = float(np.max(np.abs(r["c"] - ref["c"])))
-            o["max_dc_vs_fp64_1000_steps"] = short_horizon_error(pkg, f, c0, phi, a)
-            o["rel_energy_err_vs_fp64"] = float(np.max(np.abs(r["hist"][:, 2] - ref["hist"][:, 2])
-                                                      / np.abs(ref["hist"][:, 2])))
+            o["speedup_vs_fp64"] = ups / (sites * (a.steps + 1) / ref["seconds"])
+            o["max_dc_vs_fp64_at_timed_steps"] = float(np.max(np.abs(r["c"] - ref["c"])))
+            o["rel_energy_err_vs_fp64_at_timed_steps"] = float(np.max(np.abs(r["hist"][:, 2] - ref["hist"][:, 2])
+                                                                      / np.abs(ref["hist"][:, 2])))
+            o["horizon"] = horizon_errors(pkg, f, c0, phi, a)
         out[f] = o
-    pick = "mixed" if ("mixed" in out and out["mixed"].get("max_dc_vs_fp64", 1.0) <= 1e-5) else forms[0]
+    pick = "fp64" if "fp64" in out else forms[0]
     head = out[pick]
     dom = head["dominant"]
+    sched = head["dominant_schedule"]
+    prof_us = _rocprof_us(dom, workload, pick, sched)
     roof = dict(bound="hbm", achieved=head["dominant_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=head["dominant_gbs"] / HBM_PEAK_GBS, kernel=dom,
+                frac=head["dominant_gbs"] / HBM_PEAK_GBS, kernel=dom, schedule=sched, timing="hip_event_in_loop",
                 bytes_per_launch=head["dominant_bytes"], ms_per_launch=head["dominant_ms"],
-                traffic=_traffic(dom, f"{workload}_{pick}"), measured_copy=copy_ceiling(pkg))
+                traffic=_traffic(dom, f"{workload}_{pick}", sched),
+                rocprof_ms_per_launch=None if prof_us is None else prof_us * 1e-3,
+                frac_rocprof=None if prof_us is None else head["dominant_bytes"] / (prof_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                measured_copy=copy_ceiling(pkg))
     line = {
         "metric": "site-updates/s at 1024² (FP64 vs mixed, max|Δc|≤1e-5); % HBM roofline",
         "value": head["value"], "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps,
@@ -445,7 +522,7 @@ def main():
         s.close()
         line["cpu_baseline"] = base
         threads = min(16, len(os.sched_getaffinity(0)))
-        line["cpu_baseline_threads"] = cpu_threads_baseline(c0, phi, a, threads)
+        line["cpu_baseline_fft_threads"] = cpu_threads_baseline(c0, phi, a, threads)
     print(json.dumps(line), flush=True)
 
 

@@ -63,6 +63,7 @@ struct csg_batch {
   std::vector<hipEvent_t> join;
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
+  int64_t gevery = 1;   // history interval captured in gexec
   size_t bytes = 0;
   int nt = 0;   // non-temporal pass outputs (nontemporal_for the ensemble's footprint)
 };
@@ -110,8 +111,9 @@ void bind_out(const BatchClass& k, PassArgs& a, void* out0, void* out1) {
   a.is.out1 = out1 ? slab_stride(k, out1) : 0;
 }
 
-FinalArgs class_final(const BatchClass& k, double* hist, double dt) {
+FinalArgs class_final(const BatchClass& k, double* hist, double dt, int64_t every) {
   FinalArgs f{};
+  f.every = every;
   f.p1 = k.part;
   f.p2 = k.part + 5 * k.ops.n_row1;
   f.p3 = k.part + 5 * k.ops.n_row1 + k.ops.n_col;
@@ -131,7 +133,7 @@ FinalArgs class_final(const BatchClass& k, double* hist, double dt) {
 }
 
 // One loop iteration (core_shell.jl:90-95) of every instance of class k.
-void enqueue_class_iteration(const BatchClass& k, double dt, bool advance, hipStream_t s) {
+void enqueue_class_iteration(const BatchClass& k, double dt, bool advance, hipStream_t s, int64_t every) {
   PassArgs a = class_args(k);
   bind_in(k, a, k.S, nullptr);
   bind_out(k, a, k.A, nullptr);
@@ -152,21 +154,22 @@ void enqueue_class_iteration(const BatchClass& k, double dt, bool advance, hipSt
     bind_in(k, a, k.A, k.Bb);
     bind_out(k, a, k.S, nullptr);
     a.dt = dt;
-    a.fin = class_final(k, k.hist, dt);
+    a.fin = class_final(k, k.hist, dt, every);
     k.ops.p4(a, k.tw, s);
   } else {
-    hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(k.m), dim3(FIN_NT), 0, s, class_final(k, k.hist, dt));
+    hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(k.m), dim3(FIN_NT), 0, s, class_final(k, k.hist, dt, every));
   }
 }
 
 // `iters` loop iterations of every class, one branch per class, forked from
 // and joined back into b->stream (captured or eager).
-int enqueue_iterations(csg_batch* b, double dt, int iters, bool last_advances) {
+int enqueue_iterations(csg_batch* b, double dt, int iters, bool last_advances, int64_t every) {
   HIPCHK(hipEventRecord(b->fork, b->stream));
   for (size_t ci = 0; ci < b->cls.size(); ++ci) {
     const BatchClass& k = b->cls[ci];
     HIPCHK(hipStreamWaitEvent(k.stream, b->fork, 0));
-    for (int it = 0; it < iters; ++it) enqueue_class_iteration(k, dt, it + 1 < iters || last_advances, k.stream);
+    for (int it = 0; it < iters; ++it)
+      enqueue_class_iteration(k, dt, it + 1 < iters || last_advances, k.stream, every);
     HIPCHK(hipEventRecord(b->join[ci], k.stream));
     HIPCHK(hipStreamWaitEvent(b->stream, b->join[ci], 0));
   }
@@ -174,15 +177,15 @@ int enqueue_iterations(csg_batch* b, double dt, int iters, bool last_advances) {
   return CSG_OK;
 }
 
-int build_batch_graph(csg_batch* b, double dt) {
-  if (b->gexec && b->gdt == dt) return CSG_OK;
+int build_batch_graph(csg_batch* b, double dt, int64_t every) {
+  if (b->gexec && b->gdt == dt && b->gevery == every) return CSG_OK;
   if (b->gexec) {
     HIPCHK(hipGraphExecDestroy(b->gexec));
     b->gexec = nullptr;
   }
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_iterations(b, dt, STEPS_PER_GRAPH, true);
+  const int rc = enqueue_iterations(b, dt, STEPS_PER_GRAPH, true, every);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc) return rc;
   if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -193,6 +196,7 @@ int build_batch_graph(csg_batch* b, double dt) {
     return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   b->gdt = dt;
+  b->gevery = every;
   return CSG_OK;
 }
 
@@ -371,7 +375,9 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
     if (!b->fields_set[i]) return fail(CSG_ERR_STATE, "fields of instance " + std::to_string(i) + " not set");
   if (steps < 0 || hist_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/hist_every/dt");
   HIPCHK(hipSetDevice(b->device));
-  if (int rc = ensure_batch_hist(b, static_cast<size_t>(steps) + 1)) return rc;
+  // the device keeps rows s = 0, hist_every, 2 hist_every, ... only
+  const int64_t rows = steps / hist_every + 1;
+  if (int rc = ensure_batch_hist(b, static_cast<size_t>(rows))) return rc;
   for (BatchClass& k : b->cls) {
     std::vector<long long> init(2 * static_cast<size_t>(k.m));
     for (int q = 0; q < k.m; ++q) {
@@ -383,15 +389,14 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
   }
   int64_t done = 0;
   if (steps >= 2 * STEPS_PER_GRAPH) {
-    if (int rc = build_batch_graph(b, dt)) return rc;
+    if (int rc = build_batch_graph(b, dt, hist_every)) return rc;
     while (steps - done >= STEPS_PER_GRAPH) {
       HIPCHK(hipGraphLaunch(b->gexec, b->stream));
       done += STEPS_PER_GRAPH;
     }
   }
   // the remaining iterations, then the last history row without a step
-  if (int rc = enqueue_iterations(b, dt, static_cast<int>(steps - done) + 1, false)) return rc;
-  const int64_t rows = steps / hist_every + 1;
+  if (int rc = enqueue_iterations(b, dt, static_cast<int>(steps - done) + 1, false, hist_every)) return rc;
   std::vector<std::vector<long long>> ctr(b->cls.size());
   std::vector<std::vector<double>> h(b->cls.size());
   for (size_t ci = 0; ci < b->cls.size(); ++ci) {
@@ -400,8 +405,11 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
     HIPCHK(hipMemcpyAsync(ctr[ci].data(), k.ctr, ctr[ci].size() * sizeof(long long), hipMemcpyDeviceToHost,
                           b->stream));
     if (history) {
-      h[ci].resize(static_cast<size_t>(k.m) * k.hist_cap * 6);
-      HIPCHK(hipMemcpyAsync(h[ci].data(), k.hist, h[ci].size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+      // instance q's rows [0, rows) are at q * hist_cap * 6: one copy up to
+      // the last instance's last row
+      const size_t len = (static_cast<size_t>(k.m) - 1) * k.hist_cap * 6 + static_cast<size_t>(rows) * 6;
+      h[ci].resize(len);
+      HIPCHK(hipMemcpyAsync(h[ci].data(), k.hist, len * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     }
   }
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -421,7 +429,7 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
       const double* src = h[ci].data() + q * b->cls[ci].hist_cap * 6;
       for (int64_t r = 0; r < rows; ++r)
         for (int col = 0; col < 6; ++col)
-          history[i][r + col * rows] = src[static_cast<size_t>(r * hist_every) * 6 + col];
+          history[i][r + col * rows] = src[static_cast<size_t>(r) * 6 + col];
     }
   }
   if (any_bad)
@@ -479,7 +487,7 @@ int time_class(BatchClass& k, double dt, const int sel[4], hipEvent_t e0, hipEve
   hipGraph_t g;
   hipGraphExec_t ge = nullptr;
   HIPCHK(hipStreamBeginCapture(k.stream, hipStreamCaptureModeThreadLocal));
-  for (int it = 0; it < STEPS_PER_GRAPH; ++it) enqueue_class_iteration(k, dt, true, k.stream);
+  for (int it = 0; it < STEPS_PER_GRAPH; ++it) enqueue_class_iteration(k, dt, true, k.stream, 1);
   HIPCHK(hipStreamEndCapture(k.stream, &g));
   hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);

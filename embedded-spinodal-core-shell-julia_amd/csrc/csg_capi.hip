@@ -49,12 +49,15 @@ struct csg_ctx {
   long long* ctr = nullptr;
   double* hist = nullptr;
   size_t hist_cap = 0;    // rows
+  long long* hpin = nullptr;  // pinned host staging of ctr + history rows
+  size_t hpin_bytes = 0;
   double* scratch_row = nullptr;
   double* rate = nullptr; // (N/2+1) x N complex doubles
   bool fields_set = false;
   long long nonfinite = -1;
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
+  int64_t gevery = 1;     // history interval captured in gexec
   DevParams gp{};         // parameters captured in gexec
   size_t bytes = 0;
   int nt = 0;             // non-temporal pass outputs (nontemporal_for the footprint at creation)
@@ -107,8 +110,9 @@ double* part1(const csg_ctx* c) { return c->part; }
 double* part2(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1; }
 double* part3(const csg_ctx* c) { return c->part + 5 * c->ops.n_row1 + c->ops.n_col; }
 
-FinalArgs final_args(const csg_ctx* c, double* hist, double dt) {
+FinalArgs final_args(const csg_ctx* c, double* hist, double dt, int64_t every = 1) {
   FinalArgs f{};
+  f.every = every;
   f.p1 = part1(c);
   f.p2 = part2(c);
   f.p3 = part3(c);
@@ -150,27 +154,28 @@ void enqueue_p123(csg_ctx* c, const void* S) {
 // P4 on state `S` (read from `Sin` instead when given); with `hist` non-NULL
 // its workgroup 0 also writes the history row of the step from the P1..P3
 // partials.
-void enqueue_p4(csg_ctx* c, void* S, double dt, double* hist, const void* Sin = nullptr) {
+void enqueue_p4(csg_ctx* c, void* S, double dt, double* hist, const void* Sin = nullptr, int64_t every = 1) {
   PassArgs a = base_args(c);
   a.in0 = c->A;
   a.in1 = c->Bb;
   a.out0 = S;
   a.sin = Sin;
   a.dt = dt;
-  a.fin = final_args(c, hist, dt);
+  a.fin = final_args(c, hist, dt, every);
   c->ops.p4(a, c->tw, c->stream);
 }
 
-void enqueue_finalize(csg_ctx* c, double* hist, double dt) {
+void enqueue_finalize(csg_ctx* c, double* hist, double dt, int64_t every = 1) {
   hipLaunchKernelGGL(finalize_kernel<FIN_NT>, dim3(1), dim3(FIN_NT), 0, c->stream,
-                     final_args(c, hist, dt));
+                     final_args(c, hist, dt, every));
 }
 
-// One loop iteration of run_simulation with history (core_shell.jl:90-95).
-void enqueue_iteration(csg_ctx* c, double dt, bool advance) {
+// One loop iteration of run_simulation with history (core_shell.jl:90-95);
+// rows s with s % every == 0 are kept.
+void enqueue_iteration(csg_ctx* c, double dt, bool advance, int64_t every) {
   enqueue_p123(c, c->S);
-  if (advance) enqueue_p4(c, c->S, dt, c->hist);
-  else enqueue_finalize(c, c->hist, dt);
+  if (advance) enqueue_p4(c, c->S, dt, c->hist, nullptr, every);
+  else enqueue_finalize(c, c->hist, dt, every);
 }
 
 int upload_real(csg_ctx* c, const double* host) {
@@ -215,15 +220,15 @@ int reset_ctr(csg_ctx* c) {
 
 bool same_params(const DevParams& x, const DevParams& y) { return std::memcmp(&x, &y, sizeof(DevParams)) == 0; }
 
-int build_graph(csg_ctx* c, double dt) {
-  if (c->gexec && c->gdt == dt && same_params(c->gp, c->dp)) return CSG_OK;
+int build_graph(csg_ctx* c, double dt, int64_t every) {
+  if (c->gexec && c->gdt == dt && c->gevery == every && same_params(c->gp, c->dp)) return CSG_OK;
   if (c->gexec) {
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
   }
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_iteration(c, dt, true);
+  for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_iteration(c, dt, true, every);
   HIPCHK(hipStreamEndCapture(c->stream, &g));
   hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -232,7 +237,20 @@ int build_graph(csg_ctx* c, double dt) {
     return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   c->gdt = dt;
+  c->gevery = every;
   c->gp = c->dp;
+  return CSG_OK;
+}
+
+// Pinned host staging for ctr (2 words) and `rows` history rows.
+int ensure_hpin(csg_ctx* c, size_t rows) {
+  const size_t need = 2 * sizeof(long long) + rows * 6 * sizeof(double);
+  if (need <= c->hpin_bytes) return CSG_OK;
+  if (c->hpin) HIPCHK(hipHostFree(c->hpin));
+  c->hpin = nullptr;
+  c->hpin_bytes = 0;
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hpin), need, hipHostMallocDefault));
+  c->hpin_bytes = need;
   return CSG_OK;
 }
 
@@ -333,6 +351,7 @@ int csg_destroy(csg_ctx* c) {
     if (b) (void)hipFree(b);
   if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->hpin) (void)hipHostFree(c->hpin);
   delete c;
   return CSG_OK;
 }
@@ -378,31 +397,32 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
   if (!c->fields_set) return fail(CSG_ERR_STATE, "fields not set (csg_set_fields)");
   if (steps < 0 || hist_every < 1 || !std::isfinite(dt)) return fail(CSG_ERR_ARG, "bad steps/hist_every/dt");
   HIPCHK(hipSetDevice(c->device));
-  if (int rc = ensure_hist(c, static_cast<size_t>(steps) + 1)) return rc;
+  // the device keeps rows s = 0, hist_every, 2 hist_every, ... only
+  const int64_t rows = steps / hist_every + 1;
+  if (int rc = ensure_hist(c, static_cast<size_t>(rows))) return rc;
+  if (int rc = ensure_hpin(c, history ? static_cast<size_t>(rows) : 0)) return rc;
   if (int rc = reset_ctr(c)) return rc;
   int64_t done = 0;
   if (steps >= 2 * STEPS_PER_GRAPH) {
-    if (int rc = build_graph(c, dt)) return rc;
+    if (int rc = build_graph(c, dt, hist_every)) return rc;
     while (steps - done >= STEPS_PER_GRAPH) {
       HIPCHK(hipGraphLaunch(c->gexec, c->stream));
       done += STEPS_PER_GRAPH;
     }
   }
-  for (; done < steps; ++done) enqueue_iteration(c, dt, true);
-  enqueue_iteration(c, dt, false);
+  for (; done < steps; ++done) enqueue_iteration(c, dt, true, hist_every);
+  enqueue_iteration(c, dt, false, hist_every);
   HIPCHK(hipGetLastError());
-  long long ctr[2];
-  HIPCHK(hipMemcpyAsync(ctr, c->ctr, sizeof(ctr), hipMemcpyDeviceToHost, c->stream));
-  const int64_t rows = steps / hist_every + 1;
-  std::vector<double> h;
-  if (history) {
-    h.resize(static_cast<size_t>(steps + 1) * 6);
-    HIPCHK(hipMemcpyAsync(h.data(), c->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  }
+  long long* ctr = c->hpin;
+  double* h = reinterpret_cast<double*>(c->hpin + 2);
+  HIPCHK(hipMemcpyAsync(ctr, c->ctr, 2 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  if (history)
+    HIPCHK(hipMemcpyAsync(h, c->hist, static_cast<size_t>(rows) * 6 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (history) {
     for (int64_t r = 0; r < rows; ++r)
-      for (int col = 0; col < 6; ++col) history[r + col * rows] = h[static_cast<size_t>(r * hist_every) * 6 + col];
+      for (int col = 0; col < 6; ++col) history[r + col * rows] = h[static_cast<size_t>(r) * 6 + col];
   }
   c->nonfinite = ctr[1];
   if (ctr[1] >= 0)
@@ -1031,7 +1051,7 @@ int time_schedule(csg_ctx* c, double dt, const int sel[4], hipEvent_t e0, hipEve
     HIPCHK(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
   }
-  if (int rc = build_graph(c, dt)) return rc;
+  if (int rc = build_graph(c, dt, 1)) return rc;
   if (int rc = reset_ctr(c)) return rc;
   HIPCHK(hipGraphLaunch(c->gexec, c->stream));   // warm-up
   if (int rc = reset_ctr(c)) return rc;

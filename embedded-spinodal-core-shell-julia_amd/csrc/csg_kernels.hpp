@@ -58,6 +58,7 @@ struct FinalArgs {
   int n1, n2, n3;
   double* hist;       // history buffer (6 per row); NULL: no history
   long long* ctr;     // [0] = row index of this step, [1] = first non-finite step (-1)
+  long long every;    // rows s with s % every == 0 are kept, at hist + 6 (s / every); 0 or 1: all
   double dt;
   double spacing2;
   double inv_n2;      // 1 / N^2
@@ -219,14 +220,17 @@ __device__ void finalize_block(const FinalArgs& f, double* scratch) {
   block_reduce<NT, 7>(v, ops, scratch);
   if (threadIdx.x == 0) {
     const long long row = f.ctr[0];
-    double* h = f.hist + 6 * row;
+    const long long every = f.every > 1 ? f.every : 1;
     const bool nonfinite = v[6] > 0 || v[4] != v[4] || v[5] != v[5];
-    h[0] = static_cast<double>(row) * f.dt;
-    h[1] = v[1] * f.inv_n2;
-    h[2] = (v[0] + f.kappa * v[2] * f.inv_n2) * f.spacing2;   // core_shell.jl:80-81
-    h[3] = nonfinite ? NAN : v[4];
-    h[4] = nonfinite ? NAN : v[5];
-    h[5] = v[3] * f.spacing2;                                  // core_shell.jl:68
+    if (row % every == 0) {
+      double* h = f.hist + 6 * (row / every);
+      h[0] = static_cast<double>(row) * f.dt;
+      h[1] = v[1] * f.inv_n2;
+      h[2] = (v[0] + f.kappa * v[2] * f.inv_n2) * f.spacing2;   // core_shell.jl:80-81
+      h[3] = nonfinite ? NAN : v[4];
+      h[4] = nonfinite ? NAN : v[5];
+      h[5] = v[3] * f.spacing2;                                  // core_shell.jl:68
+    }
     if (nonfinite && f.ctr[1] < 0) f.ctr[1] = row;
     f.ctr[0] = row + 1;
   }
@@ -767,6 +771,25 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     }
     if constexpr (MODE == COL_P4) {
       fft_regs<CT, +1, N, 1, N, WAVE, E, XM, AS, BS, ST>(w, t, line, tw);
+      // The increment's k = 0 mode is exactly zero (kd = 0 there), so the
+      // real part of packed line 0 (k_x = 0) sums to zero over j and the mean
+      // composition sum_j Re S[0][j] / N^2 is conserved (core_shell.jl:74-75).
+      // An FP32 inverse transform leaves that sum at O(eps_32 |inc|); the
+      // mixed form removes it in FP64 so the mean changes only by FP64
+      // rounding.  The FP64 form needs no correction.
+      double dmean = 0.0;
+      if constexpr (!std::is_same<CT, double>::value) {
+        if (lb == 0) {   // uniform: the workgroup that holds line k_x = 0
+          double r[1] = {0.0};
+          if (kx == 0) {
+#pragma unroll
+            for (int j = 0; j < E; ++j) r[0] += static_cast<double>(w[0][j].x);
+          }
+          const int ops[1] = {RED_SUM};
+          block_reduce<NT, 1>(r, ops, scratch);
+          dmean = (kx == 0) ? r[0] / N : 0.0;
+        }
+      }
       // S += IFFT_j(inc)
       cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
       if constexpr (!Cfg::PF) {
@@ -777,11 +800,10 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
 #pragma unroll
       for (int j = 0; j < E; ++j) {
         cplx<TS> s = sp[j];
-        s.x += static_cast<TS>(w[0][j].x);
+        s.x += static_cast<TS>(static_cast<double>(w[0][j].x) - dmean);
         s.y += static_cast<TS>(w[0][j].y);
         store_out(&S[base + t + TPL * j], s, a.nt);
       }
-
     }
   }
 }

@@ -148,3 +148,65 @@ def test_ensemble_nonfinite_member_does_not_stop_others(pkg):
         c1, _ = _single(pkg, ms[i], fields[i], 40, "fp64")
         assert np.array_equal(ens.field(i), c1)
     ens.close()
+
+
+def _c3_members(pkg):
+    """Two members of each ensemble size (128, 256, 384, 512): R = 0.3 N,
+    w = 2, P and chi within +-30 % of the defaults, distinct seeds."""
+    d = pkg.Parameters()
+    spec = [(128, 0.72, 1.25), (128, 1.28, 0.75), (256, 0.85, 1.1), (256, 1.15, 0.9),
+            (384, 0.7, 1.3), (384, 1.3, 0.7), (512, 0.95, 1.2), (512, 1.2, 0.8)]
+    return [pkg.Member(n=n, radius=0.3 * n, width=2.0, seed=7001 + i,
+                       p=dataclasses.replace(d, P=d.P * fp, chi=d.chi * fc))
+            for i, (n, fp, fc) in enumerate(spec)]
+
+
+@pytest.mark.gpu
+def test_ensemble_c3_sizes_match_oracle(pkg):
+    """Ensemble of all four C3 sizes: FP64 members against the oracle's
+    run_simulation (50 steps, <= 1e-12), mixed members against their FP64
+    twins, and the mixed mean composition conserved at FP64 level."""
+    ms = _c3_members(pkg)
+    steps = 50
+    say(f"C3 sizes ensemble vs oracle, {steps} steps, sizes {[m.n for m in ms]}")
+    fields = [pkg.member_fields(m) for m in ms]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        out[prec] = pkg.run_ensemble([f[0] for f in fields], [f[1] for f in fields], steps=steps,
+                                     params=[m.p for m in ms], precision=prec)
+    for i, m in enumerate(ms):
+        c0, phi = fields[i]
+        po = o.Parameters(**dataclasses.asdict(m.p))
+        cr, hr = o.run_simulation_fast(np.asarray(c0), np.asarray(phi), steps=steps, p=po)
+        c64, h64 = out["fp64"][i]
+        cmx, hmx = out["mixed"][i]
+        assert np.max(np.abs(c64 - cr)) < 1e-12, (i, m.n)
+        assert np.max(np.abs(h64 - hr) / np.maximum(1.0, np.abs(hr))) < 1e-12, (i, m.n)
+        assert np.max(np.abs(cmx - c64)) < 1e-5, (i, m.n)
+        assert np.max(np.abs(hmx[:, 2] - h64[:, 2]) / np.abs(h64[:, 2])) < 1e-5, (i, m.n)
+        assert np.max(np.abs(hmx[:, 1] - hmx[0, 1])) <= 1e-13, (i, m.n)
+
+
+@pytest.mark.gpu
+def test_ensemble_c3_full_workload_properties(pkg):
+    """The full uneven ensemble (256 members, N in {128, 256, 384, 512},
+    2000 steps, mixed form): every member finite, its mean composition kept
+    to 1e-13 and its energy non-increasing (checks.jl:19-20 per member)."""
+    ms = pkg.ensemble_members(256)
+    say("C3 full ensemble, 256 members, 2000 steps, mixed")
+    fields = [pkg.member_fields(m) for m in ms]
+    ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision="mixed")
+    for i, (c0, phi) in enumerate(fields):
+        ens.set_fields(i, c0, phi)
+    hs = ens.run(0.01, 2000, history_every=10)
+    for i, (m, h) in enumerate(zip(ms, hs)):
+        c0 = fields[i][0]
+        assert h.shape == (201, 6)
+        assert np.all(np.isfinite(h)), i
+        assert abs(h[0, 1] - np.mean(c0)) < 1e-14, i
+        assert np.max(np.abs(h[:, 1] - h[0, 1])) <= 1e-13, (i, m.n)
+        assert np.max(np.diff(h[:, 2])) < 1e-8, (i, m.n)
+    for i in (0, 100, 255):
+        c = ens.field(i)
+        assert np.all(np.isfinite(c)) and c.shape == (ms[i].n, ms[i].n)
+    ens.close()

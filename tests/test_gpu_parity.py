@@ -173,6 +173,35 @@ def test_known_answer_loop_256(pkg):
     assert np.max(np.abs(final - fr)) < 1e-10
 
 
+def test_stated_field_256_r80_w2(pkg):
+    """The plain-method workload as stated (256^2, R=80, w=2, dt=0.01, 1000
+    steps, history every 10 steps), FP64, against the oracle field; the
+    checks.jl:19-20 assertions on the history."""
+    say("256^2 R=80 w=2 1000 steps fp64 vs oracle")
+    c0, phi = _fields(pkg, 256, radius=80.0, width=2.0)
+    final, hist = pkg.run_simulation(c0, phi, dt=0.01, steps=1000, history_every=10)
+    fr, hr = o.run_simulation_fast(c0, phi, dt=0.01, steps=1000)
+    assert hist.shape == (101, 6)
+    assert np.max(np.abs(final - fr)) < 1e-10
+    assert np.max(np.abs(hist - hr[::10]) / np.maximum(1.0, np.abs(hr[::10]))) < 1e-10
+    assert np.max(np.abs(hist[:, 1] - c0.mean())) < 1e-13
+    assert np.max(np.diff(hist[:, 2])) < 1e-8
+
+
+@pytest.mark.parametrize("n,steps", [(512, 2000), (1024, 200)])
+def test_mixed_mass_drift_fp64_level(pkg, n, steps):
+    """Mixed form: the mean composition changes only by FP64 rounding (the
+    increment's k = 0 mode is removed in FP64), as checks.jl:19 asks of the
+    FP64 path, with a 10x tighter bound; energy non-increasing (checks.jl:20)."""
+    say(f"mixed mass drift {n}^2, {steps} steps")
+    c0, phi = _fields(pkg, n, radius=0.3 * n, width=2.0)
+    _, hist = pkg.run_simulation(c0, phi, dt=0.01, steps=steps, precision="mixed")
+    drift = np.max(np.abs(hist[:, 1] - hist[0, 1]))
+    assert drift <= 1e-13, drift
+    assert abs(hist[0, 1] - c0.mean()) < 1e-14
+    assert np.max(np.diff(hist[:, 2])) < 1e-8
+
+
 def test_checks_jl_identities_on_gpu(pkg):
     say("checks.jl identities against the GPU backend")
     c, phi = _fields(pkg, 64, radius=16.0)
