@@ -23,6 +23,9 @@ CSG_ERR_STATE = 5
 CSG_FP64 = 0
 CSG_MIXED = 1
 
+CSG_NOALLOC_MAX_N = 1024
+CSG_NOALLOC_SCRATCH_BYTES = 26009344
+
 EXPORTS = (
     "csg_default_params", "csg_create", "csg_destroy", "csg_set_fields", "csg_run",
     "csg_step", "csg_get_field", "csg_nonfinite_step", "csg_chemical_potential",
@@ -33,7 +36,7 @@ EXPORTS = (
     "csg_batch_get_field", "csg_batch_nonfinite_step", "csg_batch_device_bytes",
     "csg_schedule_info", "csg_get_schedule", "csg_set_schedule", "csg_autotune",
     "csg_batch_get_schedule", "csg_batch_set_schedule", "csg_batch_autotune", "csg_set_params",
-    "csg_copy_bandwidth",
+    "csg_copy_bandwidth", "csg_step_noalloc",
 )
 
 
@@ -101,6 +104,7 @@ def lib():
         "csg_batch_autotune": (ctypes.c_int, [_VP, dbl, i32]),
         "csg_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(CsgParams)]),
         "csg_copy_bandwidth": (ctypes.c_int, [i32, i64, i32, _D]),
+        "csg_step_noalloc": (ctypes.c_int, [i32, dbl, ctypes.POINTER(CsgParams), i32, i32, dbl, i64, _VP, _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -30,6 +30,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 UNITS = [
     ("csg_capi.hip", []),
     ("csg_batch.hip", []),
+    ("csg_noalloc.hip", []),
 ] + [(f"csg_ops_{prec}_{group}.hip", []) for prec in ("fp64", "mixed") for group in ("small", "mid", "big", "pow3")]
 
 

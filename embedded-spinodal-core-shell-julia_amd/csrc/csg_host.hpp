@@ -36,13 +36,13 @@ inline std::string unsupported_size_message(int n) {
          " not supported (N = 2^m with 16 <= N <= 4096, or N = 3*2^m with 48 <= N <= 768)";
 }
 
-// Twiddle buffer of an N-point grid (layout: stab_offset): W_N^m, m in
-// [0, N), from extended-precision sines and cosines rounded once, then each
-// stage-table entry taken from the first half of that table (negated past
-// the half circle), so both twiddle paths use the same values.
+// Twiddle buffer of an N-point grid (layout: stab_offset), written to
+// w[0 .. twiddle_len(n)): W_N^m, m in [0, N), from extended-precision sines
+// and cosines rounded once, then each stage-table entry taken from the first
+// half of that table (negated past the half circle), so both twiddle paths
+// use the same values.  Allocates nothing.
 template <class T>
-std::vector<cplx<T>> make_twiddles(int n) {
-  std::vector<cplx<T>> w(twiddle_len(n));
+void fill_twiddles(int n, cplx<T>* w) {
   for (int m = 0; m < n; ++m) {
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / n;
     w[m] = cplx<T>{static_cast<T>(cosl(ang)), static_cast<T>(sinl(ang))};
@@ -61,6 +61,12 @@ std::vector<cplx<T>> make_twiddles(int n) {
       }
     }
   }
+}
+
+template <class T>
+std::vector<cplx<T>> make_twiddles(int n) {
+  std::vector<cplx<T>> w(twiddle_len(n));
+  fill_twiddles<T>(n, w.data());
   return w;
 }
 

@@ -225,6 +225,31 @@ int csg_batch_get_schedule(const csg_batch* b, int32_t n, int32_t* sel);
 int csg_batch_set_schedule(csg_batch* b, int32_t n, const int32_t* sel);
 int csg_batch_autotune(csg_batch* b, double dt, int32_t rounds);
 
+/* ---- allocation-free stepping ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code between graph captures, tight serving loops, harnesses that
+ * forbid hipMalloc): nsteps applications of step (core_shell.jl:72-76) to a
+ * composition field already in device memory, with the same kernels, default
+ * schedule and twiddles as csg_create + csg_set_fields + csg_step +
+ * csg_get_field, so the result is identical bit for bit.
+ *
+ * c_dev: device, column-major N x N float64, advanced in place.  phi_dev:
+ * device, column-major N x N float64.  p: NULL for the defaults.  stream: a
+ * hipStream_t, or NULL for the null stream; the call returns after the work
+ * on it has finished.
+ *
+ * Everything else the kernels use (packed state, work arrays, partials,
+ * twiddles) lives in a static device array of CSG_NOALLOC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends; the call
+ * allocates nothing.  Grids above CSG_NOALLOC_MAX_N are rejected with
+ * CSG_ERR_ARG.  Calls on one device share that array: the library serialises
+ * them with a host lock within a process; work a caller enqueues on the same
+ * device concurrently is not affected (the array is private to the library). */
+#define CSG_NOALLOC_MAX_N 1024
+#define CSG_NOALLOC_SCRATCH_BYTES 26009344
+int csg_step_noalloc(int32_t n, double spacing, const csg_params* p, int32_t precision, int32_t device,
+                     double dt, int64_t nsteps, double* c_dev, const double* phi_dev, void* stream);
+
 const char* csg_last_error(void);
 
 #ifdef __cplusplus
