@@ -13,6 +13,7 @@ GPU energy.  FP64 only.
 """
 from __future__ import annotations
 
+import collections
 import dataclasses
 import math
 
@@ -69,8 +70,11 @@ def _energy_param_grads(s: Session, c, p: Parameters):
 
 
 # One FP64 context per (n, spacing, device), kept between calls: the checkpoint
-# and segment buffers and the captured adjoint graphs are then reused.
-_contexts: dict = {}
+# and segment buffers and the captured adjoint graphs are then reused.  At
+# most MAX_CONTEXTS are kept (their checkpoints can hold gigabytes); the least
+# recently used one is closed first.
+MAX_CONTEXTS = 2
+_contexts: "collections.OrderedDict" = collections.OrderedDict()
 
 
 def _context(n: int, spacing: float, p: Parameters, device: int) -> Session:
@@ -78,8 +82,13 @@ def _context(n: int, spacing: float, p: Parameters, device: int) -> Session:
     s = _contexts.get(key)
     if s is None:
         s = _contexts[key] = Session(n, spacing, p, "fp64", device)
-    elif s.p != p:
-        s.set_params(p)
+        while len(_contexts) > MAX_CONTEXTS:
+            _, old = _contexts.popitem(last=False)
+            old.close()
+    else:
+        _contexts.move_to_end(key)
+        if s.p != p:
+            s.set_params(p)
     return s
 
 
@@ -94,7 +103,8 @@ def gradient(c0, phi, target, *, dt: float = 0.01, steps: int = 1000, lam: float
     rc = lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K))
     if rc == CSG_ERR_HIP and ckpt_every is None and K == 1:
         # not enough free device memory to keep every state: checkpoint instead
-        # (buffers already allocated are kept and reused)
+        # (the failed call has freed the checkpoint buffers it allocated; the
+        # retry keeps only the ~sqrt(steps) it needs)
         K = max(1, int(round(math.sqrt(steps))))
         s.set_fields(c0, phi)
         rc = lib().csg_adjoint_forward(s.handle, float(dt), int(steps), int(K))

@@ -12,6 +12,7 @@ reproduce the reference: ``precision`` ("fp64" or "mixed"), ``history_every``
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import dataclasses
 import os
@@ -225,15 +226,27 @@ def _key(phi: np.ndarray):
     return (phi.shape, hash(phi.tobytes()))
 
 
-_sessions: dict = {}
+# Device contexts reused by the function API: one per (n, spacing,
+# precision, device), with the Parameters swapped in place (csg_set_params,
+# no reallocation) when a call brings others; at most MAX_SESSIONS are kept,
+# the least recently used one is closed first.
+MAX_SESSIONS = 8
+_sessions: "collections.OrderedDict" = collections.OrderedDict()
 
 
 def _session(n, spacing, p, precision, device=0) -> Session:
-    k = (int(n), float(spacing), p, precision, int(device))
+    k = (int(n), float(spacing), precision, int(device))
     s = _sessions.get(k)
     if s is None:
         s = Session(n, spacing, p, precision, device)
         _sessions[k] = s
+        while len(_sessions) > MAX_SESSIONS:
+            _, old = _sessions.popitem(last=False)
+            old.close()
+    else:
+        _sessions.move_to_end(k)
+        if s.p != p:
+            s.set_params(p)
     return s
 
 

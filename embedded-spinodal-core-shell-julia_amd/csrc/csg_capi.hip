@@ -84,7 +84,22 @@ struct csg_ctx {
 
 namespace {
 
+// Test hook: CSG_TEST_DEVICE_BUDGET=<bytes> in the environment makes any
+// allocation that would take a context past that many bytes fail as an
+// out-of-memory hipMalloc would, so fallback paths can be exercised without
+// exhausting the device.
+bool over_test_budget(const csg_ctx* c, size_t bytes) {
+  const char* e = std::getenv("CSG_TEST_DEVICE_BUDGET");
+  if (!e || !*e) return false;
+  const unsigned long long cap = std::strtoull(e, nullptr, 10);
+  return c->bytes + bytes > cap;
+}
+
 int dmalloc(csg_ctx* c, void** p, size_t bytes) {
+  if (over_test_budget(c, bytes)) {
+    *p = nullptr;
+    return fail(CSG_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): over CSG_TEST_DEVICE_BUDGET");
+  }
   const hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     // clear the runtime's last error, so a caller that falls back to a
@@ -782,12 +797,59 @@ int csg_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs)
 // Adjoint: checkpointed forward run and backward sweep (FP64 contexts).
 namespace {
 
+void free_back(csg_ctx* c, std::vector<void*>& v, size_t keep, size_t bytes) {
+  while (v.size() > keep) {
+    (void)hipFree(v.back());
+    c->bytes -= bytes;
+    v.pop_back();
+  }
+}
+
+// Exactly `count` buffers of `bytes` in v.  On an allocation failure the
+// buffers this call allocated are freed again, so a caller can retry with
+// fewer (keeping every state -> checkpoints).  Callers drop the captured
+// adjoint graphs before the count can shrink.
 int ensure_bufs(csg_ctx* c, std::vector<void*>& v, size_t count, size_t bytes) {
+  const size_t had = v.size();
+  if (had > count) {
+    free_back(c, v, count, bytes);
+    return CSG_OK;
+  }
   while (v.size() < count) {
     void* p = nullptr;
-    if (int rc = dmalloc(c, &p, bytes)) return rc;
+    if (int rc = dmalloc(c, &p, bytes)) {
+      free_back(c, v, had, bytes);
+      return rc;
+    }
     v.push_back(p);
   }
+  return CSG_OK;
+}
+
+// The backward sweep's work buffers: adjoint field, real-space scratch,
+// parameter-term partials.
+int ensure_adjoint_work(csg_ctx* c) {
+  const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
+  if (!c->G)
+    if (int rc = dmalloc(c, &c->G, bytes)) return rc;
+  if (!c->Xr)
+    if (int rc = dmalloc(c, &c->Xr, bytes)) return rc;
+  const size_t nacc = 2 * static_cast<size_t>(c->ops.n_adj_col) + c->ops.n_adj_row;
+  if (!c->adj_acc)
+    if (int rc = dmalloc(c, reinterpret_cast<void**>(&c->adj_acc), nacc * sizeof(double))) return rc;
+  return CSG_OK;
+}
+
+// Segment buffers for checkpoint interval K (none when every state is kept).
+int ensure_segments(csg_ctx* c, int64_t K) {
+  const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
+  if (K > 1) {
+    // the recompute and backward graphs bind the segment buffers
+    if (c->seg.size() != static_cast<size_t>(K) + 1) drop_adjoint_graphs(c);
+    return ensure_bufs(c, c->seg, static_cast<size_t>(K) + 1, bytes);
+  }
+  // every state kept: no segments; a change of K has dropped the graphs that bound them
+  free_back(c, c->seg, 0, bytes);
   return CSG_OK;
 }
 
@@ -881,10 +943,17 @@ int csg_adjoint_forward(csg_ctx* c, double dt, int64_t steps, int64_t ckpt_every
   const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
   const int64_t K = ckpt_every, nfull = steps / K;
   const size_t nck = static_cast<size_t>(nfull) + 1;
+  // captured graphs bind the checkpoint buffers: drop them before any change
+  if (c->adj_gK != K || c->adj_gdt != dt || !same_params(c->adj_gp, c->dp) || (K == 1 && c->adj_all_T != steps) ||
+      c->ckpt.size() != nck)
+    drop_adjoint_graphs(c);
+  c->adj_steps = -1;   // no valid forward run until this one completes
+  // everything the backward sweep needs is allocated here, so a run that does
+  // not fit fails now (and can be retried with checkpoints) rather than there
+  if (int rc = ensure_adjoint_work(c)) return rc;
+  if (int rc = ensure_segments(c, K)) return rc;
   if (int rc = ensure_bufs(c, c->ckpt, nck, bytes)) return rc;
   c->adj_dt = dt;
-  if (c->adj_gK != K || c->adj_gdt != dt || !same_params(c->adj_gp, c->dp) || (K == 1 && c->adj_all_T != steps))
-    drop_adjoint_graphs(c);
   if (K == 1) {
     // every state kept: each step writes its result straight into the next
     // checkpoint (out-of-place P4), so the backward sweep needs no recompute
@@ -948,13 +1017,10 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
   const int64_t T = c->adj_steps, K = c->adj_every;
   const double dt = c->adj_dt;
   const size_t bytes = static_cast<size_t>(c->n) * c->n * sizeof(double);
-  if (!c->G && dmalloc(c, &c->G, bytes)) return CSG_ERR_HIP;
-  if (!c->Xr && dmalloc(c, &c->Xr, bytes)) return CSG_ERR_HIP;
+  if (int rc = ensure_adjoint_work(c)) return rc;   // allocated by the forward run already
+  if (int rc = ensure_segments(c, K)) return rc;
   const int na = c->ops.n_adj_col, nr = c->ops.n_adj_row;
   const size_t nacc = 2 * static_cast<size_t>(na) + nr;
-  if (!c->adj_acc && dmalloc(c, reinterpret_cast<void**>(&c->adj_acc), nacc * sizeof(double))) return CSG_ERR_HIP;
-  if (K > 1)
-    if (int rc = ensure_bufs(c, c->seg, static_cast<size_t>(K) + 1, bytes)) return rc;
   HIPCHK(hipMemsetAsync(c->adj_acc, 0, nacc * sizeof(double), c->stream));
   // gbar_T into the state's packed line layout
   if (int rc = upload_real(c, dL_dcT)) return rc;

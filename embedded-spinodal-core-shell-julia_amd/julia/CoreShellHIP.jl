@@ -36,9 +36,15 @@ end
 lasterror() = unsafe_string(ccall((:csg_last_error, libcsg), Cstring, ()))
 check(rc) = rc == CSG_OK || error("csg: " * lasterror())
 
+# One device context (csg_create).  `generation` counts field uploads and
+# parameter changes, so a holder of checkpoints (the adjoint rule) can tell
+# whether the context has been used for something else since.
 mutable struct Context
     ptr::Ptr{Cvoid}
     n::Int
+    p::Parameters
+    phi_hash::UInt
+    generation::Int
     function Context(n::Integer, spacing::Real, p::Parameters, precision::Symbol, device::Integer)
         out = Ref{Ptr{Cvoid}}(C_NULL)
         prec = precision === :fp64 ? Cint(0) : precision === :mixed ? Cint(1) :
@@ -47,15 +53,37 @@ mutable struct Context
         check(ccall((:csg_create, libcsg), Cint,
                     (Ref{Ptr{Cvoid}}, Int32, Float64, Ref{Parameters}, Int32, Int32),
                     out, n, spacing, pref, prec, device))
-        ctx = new(out[], Int(n))
-        finalizer(c -> ccall((:csg_destroy, libcsg), Cint, (Ptr{Cvoid},), c.ptr), ctx)
+        ctx = new(out[], Int(n), p, UInt(0), 0)
+        finalizer(destroy!, ctx)
         return ctx
     end
+end
+
+# Free the device context now (also the finalizer); idempotent.
+function destroy!(ctx::Context)
+    if ctx.ptr != C_NULL
+        ccall((:csg_destroy, libcsg), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    return nothing
+end
+
+# Swap the Parameters in place (csg_set_params: nothing is reallocated).
+function set_params!(ctx::Context, p::Parameters)
+    ctx.p == p && return ctx
+    pref = Ref(p)
+    check(ccall((:csg_set_params, libcsg), Cint, (Ptr{Cvoid}, Ref{Parameters}), ctx.ptr, pref))
+    ctx.p = p
+    ctx.generation += 1
+    return ctx
 end
 
 function set_fields!(ctx::Context, c0::Matrix{Float64}, phi::Matrix{Float64})
     GC.@preserve c0 phi check(ccall((:csg_set_fields, libcsg), Cint,
                                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.ptr, c0, phi))
+    ctx.phi_hash = hash(phi)
+    ctx.generation += 1
+    return ctx
 end
 
 function field(ctx::Context)
@@ -87,14 +115,41 @@ function initial_fields(n; spacing=1.0, radius=70.0, width=4.0, seed=5749)
     return c, phi
 end
 
-const contexts = Dict{Any,Context}()
-function context(phi, spacing, p, precision, device)
-    key = (size(phi, 1), Float64(spacing), p, precision, device, hash(phi))
-    get!(contexts, key) do
-        ctx = Context(size(phi, 1), spacing, p, precision, device)
-        set_fields!(ctx, Matrix{Float64}(phi), Matrix{Float64}(phi))
-        ctx
+# Contexts reused by the function API: one per (n, spacing, precision,
+# device), Parameters swapped in place and phi re-uploaded only when it
+# changes; at most MAX_CONTEXTS are kept, the least recently used is freed
+# first.  `cache` and `order` let the adjoint rule keep its own, smaller set.
+const MAX_CONTEXTS = 8
+const ContextKey = Tuple{Int,Float64,Symbol,Int}
+const contexts = Dict{ContextKey,Context}()
+const context_order = ContextKey[]   # least recently used first
+
+function cached_context(n, spacing, p, precision, device; cache=contexts, order=context_order,
+                        limit=MAX_CONTEXTS)
+    key = (Int(n), Float64(spacing), precision, Int(device))
+    ctx = get(cache, key, nothing)
+    if ctx === nothing || ctx.ptr == C_NULL
+        while length(order) >= limit
+            destroy!(pop!(cache, popfirst!(order)))
+        end
+        filter!(!=(key), order)
+        ctx = Context(n, spacing, p, precision, device)
+        cache[key] = ctx
+    else
+        filter!(!=(key), order)
+        set_params!(ctx, p)
     end
+    push!(order, key)
+    return ctx
+end
+
+function context(phi, spacing, p, precision, device)
+    ctx = cached_context(size(phi, 1), spacing, p, precision, device)
+    if ctx.phi_hash != hash(phi)
+        m = Matrix{Float64}(phi)
+        set_fields!(ctx, m, m)
+    end
+    return ctx
 end
 
 # geometry (core_shell.jl:39-43); the shape keeps phi for the device calls.
@@ -148,8 +203,8 @@ end
 
 # step (core_shell.jl:72-76)
 function step(c, shape, grid, p, dt; precision=:fp64, device=0)
-    ctx = Context(size(c, 1), grid.spacing, p, precision, device)
-    set_fields!(ctx, Matrix{Float64}(c), shape.phi)
+    ctx = cached_context(size(c, 1), grid.spacing, p, precision, device)
+    set_fields!(ctx, Matrix{Float64}(c), Matrix{Float64}(shape.phi))
     check(ccall((:csg_step, libcsg), Cint, (Ptr{Cvoid}, Float64, Int64), ctx.ptr, dt, 1))
     return field(ctx)
 end
@@ -157,7 +212,7 @@ end
 # run_simulation (core_shell.jl:84-103)
 function run_simulation(c0, phi; dt=0.01, steps=1000, spacing=1.0, p=Parameters(),
                         directory=nothing, precision=:fp64, history_every=1, device=0)
-    ctx = Context(size(c0, 1), spacing, p, precision, device)
+    ctx = cached_context(size(c0, 1), spacing, p, precision, device)
     set_fields!(ctx, Matrix{Float64}(c0), Matrix{Float64}(phi))
     rows = steps ÷ history_every + 1
     history = zeros(rows, 6)

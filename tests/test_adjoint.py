@@ -196,3 +196,33 @@ def test_gpu_kept_states_survive_schedule_change(pkg):
     assert np.max(np.abs(out[1][0] - out[0][0])) <= 1e-10 * np.max(np.abs(out[0][0]))
     assert np.all(np.abs(out[1][1] - out[0][1]) <= 1e-10 * np.abs(out[0][1]))
 
+
+
+@pytest.mark.gpu
+def test_keep_every_state_falls_back_to_checkpoints(pkg, monkeypatch):
+    """When keeping every state does not fit (forced here with the library's
+    test-only device byte budget), the failed forward run frees the
+    checkpoints it allocated and gradient() retries with ~sqrt(steps)
+    checkpoints within the budget; the gradient equals the one computed with
+    that interval directly."""
+    n, steps, dt = 128, 400, 0.01
+    c0, phi = pkg.initial_fields(n, radius=38.4, width=2.0)
+    tgt = np.asfortranarray(np.asarray(c0) + 0.01 * np.sin(np.arange(n * n).reshape(n, n)))
+    say("adjoint: keep-every-state fallback under a 24 MiB budget")
+    ref = pkg.adjoint.gradient(c0, phi, tgt, dt=dt, steps=steps, ckpt_every=20)
+    for k in list(pkg.adjoint._contexts):
+        pkg.adjoint._contexts.pop(k).close()
+    budget = 24 << 20
+    assert (steps + 1) * n * n * 8 > budget            # every state cannot fit
+    assert pkg.adjoint.default_ckpt_every(n, steps) == 1
+    monkeypatch.setenv("CSG_TEST_DEVICE_BUDGET", str(budget))
+    try:
+        g = pkg.adjoint.gradient(c0, phi, tgt, dt=dt, steps=steps)
+        ctx = pkg.adjoint._contexts[(n, 1.0, 0)]
+        assert ctx.device_bytes() <= budget
+    finally:
+        for k in list(pkg.adjoint._contexts):
+            pkg.adjoint._contexts.pop(k).close()
+    assert np.array_equal(g.dc0, ref.dc0)
+    assert (g.dkappa, g.dP, g.dchi) == (ref.dkappa, ref.dP, ref.dchi)
+    assert g.loss == ref.loss

@@ -153,3 +153,35 @@ def test_bench_refuses_more_than_one_gpu():
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 2 and r.stdout == ""
     assert "one GPU only" in r.stderr
+
+
+def test_function_api_context_cache_is_bounded(pkg, monkeypatch):
+    """The function API reuses one context per (n, spacing, precision,
+    device), swapping Parameters in place, and keeps at most MAX_SESSIONS
+    (host logic; a stand-in Session records what the library would do)."""
+    from csg_amd import core_shell as cs
+    made, closed = [], []
+
+    class FakeSession:
+        def __init__(self, n, spacing, p, precision, device):
+            self.key, self.p, self.swaps = (n, spacing, precision, device), p, 0
+            made.append(self)
+
+        def set_params(self, p):
+            self.p, self.swaps = p, self.swaps + 1
+
+        def close(self):
+            closed.append(self)
+
+    monkeypatch.setattr(cs, "Session", FakeSession)
+    monkeypatch.setattr(cs, "_sessions", type(cs._sessions)())
+    for i in range(50):
+        p = pkg.Parameters(P=3.5 + 0.06 * i, chi=0.35 + 0.006 * i)
+        s = cs._session(64, 1.0, p, "fp64")
+        assert s.p == p
+    assert len(made) == 1 and made[0].swaps == 49 and len(cs._sessions) == 1
+    for n in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
+        cs._session(n, 1.0, pkg.Parameters(), "mixed")
+    assert len(cs._sessions) == cs.MAX_SESSIONS
+    assert len(made) - len(closed) == cs.MAX_SESSIONS
+    assert all(s.key[0] in (16, 32, 48, 64, 96) for s in closed)
