@@ -226,7 +226,7 @@ def run_form(pkg, form, c0, phi, a):
     ms_pass = s.time_passes(a.dt, 50)
     sched = list(s.schedule)
     info = s.schedule_info()
-    resolved = [_sched_str(info[q][sched[q]][k] for k in ("E", "W", "G", "PF", "X", "T")) for q in range(4)]
+    resolved = [_sched_str(info[q][sched[q]][k] for k in ("E", "W", "G", "PF", "X", "T", "S")) for q in range(4)]
     s.close()
     return dict(seconds=t1 - t0, hist=hist, c=c, ms_pass=ms_pass, schedule=sched, resolved=resolved, tune=tune,
                 reps=reps, rep_steps=k)

@@ -135,8 +135,10 @@ class Session:
     def schedule_info(self) -> list:
         """Per pass, the compiled candidates as dicts (E = elements per thread,
         W = rows or lines per workgroup, G = tile-load group, PF = prefetch,
-        X = FFT exchange layout, T = 1 for per-stage twiddle tables)."""
-        keys = ("E", "W", "G", "PF", "X", "T")
+        X = FFT exchange layout, T = 1 for per-stage twiddle tables, S = 1
+        when P3 gives each gradient component its own workgroups, 0 when one
+        workgroup transforms both; 0 for the other passes)."""
+        keys = ("E", "W", "G", "PF", "X", "T", "S")
         out = []
         for q in range(4):
             cnt = ctypes.c_int32()

@@ -255,12 +255,6 @@ constexpr int row_rows() {
   return b;
 }
 
-// P3 gives each gradient component its own workgroups (CSG_P3_SPLIT=1,
-// more parallelism) or transforms both in one workgroup (phi read once).
-#ifndef CSG_P3_SPLIT
-#define CSG_P3_SPLIT 1
-#endif
-
 // Schedule of a pass kernel (0 = the default rule):
 //   E   elements per thread (register tile; sets the radix plan),
 //   W   rows (row pass) or lines (column pass) per workgroup,
@@ -271,10 +265,13 @@ constexpr int row_rows() {
 //   X   FFT exchange layout (fft_regs XM: 0 one array at a time, 1 all
 //       arrays together, 2 all together with two buffer sets); -1 = 0,
 //   T   FFT twiddles: 1 per-stage tables copied to LDS (lanes read
-//       consecutive entries), 0 the half table; -1 = 0.
-template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1, int X_ = -1, int T_ = -1>
+//       consecutive entries), 0 the half table; -1 = 0,
+//   S   P3 only: 1 gives each gradient component its own workgroups (phi is
+//       then read once per component), 0 transforms both components of W
+//       rows in one workgroup (phi read once).
+template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1, int X_ = -1, int T_ = -1, int S_ = 1>
 struct Sched {
-  static constexpr int E = E_, W = W_, G = G_, PF = PF_, X = X_, T = T_;
+  static constexpr int E = E_, W = W_, G = G_, PF = PF_, X = X_, T = T_, S = S_;
 };
 
 template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
@@ -282,7 +279,7 @@ struct RowCfg {
   static constexpr int L = N / 2;
   static constexpr int E = SC::E ? SC::E : fft_E(L);
   static constexpr int TPL = L / E;
-  static constexpr bool SPLIT = (MODE == ROW_P3) && CSG_P3_SPLIT;
+  static constexpr bool SPLIT = (MODE == ROW_P3) && SC::S != 0;
   static constexpr int NARR = (MODE == ROW_P3 && !SPLIT) ? 2 : 1;   // arrays per workgroup
   static constexpr int B = SC::W ? SC::W : row_rows<N, CT, NARR, E>();
   static constexpr int G = SC::G ? SC::G : E;

@@ -11,13 +11,13 @@ namespace csg {
 using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
 
 constexpr int MAX_CANDS = 5;
-constexpr int SCHED_FIELDS = 6;   // E, W, G, PF, X, T
+constexpr int SCHED_FIELDS = 7;   // E, W, G, PF, X, T, S
 
 // Compiled schedules of one step pass.
 struct PassCands {
   LaunchFn fn[MAX_CANDS] = {};
   int grid[MAX_CANDS] = {};
-  int sched[MAX_CANDS][SCHED_FIELDS] = {};   // resolved E, W, G, PF, X, T
+  int sched[MAX_CANDS][SCHED_FIELDS] = {};   // resolved E, W, G, PF, X, T, S
   int n = 0;
 };
 
@@ -96,14 +96,15 @@ void add_cand(PassCands& pc) {
     using Cfg = RowCfg<T, TIN, T, N, MODE, SC>;
     pc.fn[i] = launch_row<T, TIN, T, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0};
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0,
+                                 PASS == 3 ? (Cfg::SPLIT ? 1 : 0) : 0};
     for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   } else {
     constexpr int MODE = PASS == 2 ? COL_P2 : COL_P4;
     using Cfg = ColCfg<T, double, N, MODE, SC>;
     pc.fn[i] = launch_col<T, double, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::C, 0, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0};
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::C, 0, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0, 0};
     for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   }
 }

@@ -170,7 +170,7 @@ int csg_set_params(csg_ctx* ctx, const csg_params* p);
  * not bit for bit).
  *
  * csg_schedule_info: count of candidates of `pass`; `sched` (may be NULL,
- * else 6 * count int32) receives each candidate's [elements per thread, rows
+ * else 7 * count int32) receives each candidate's [elements per thread, rows
  * or lines per workgroup, tile-load group, prefetch, FFT exchange layout,
  * twiddle source (1 = per-stage tables)]. */
 int csg_schedule_info(const csg_ctx* ctx, int32_t pass, int32_t* count, int32_t* sched);
