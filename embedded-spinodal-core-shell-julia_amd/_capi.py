@@ -36,7 +36,7 @@ EXPORTS = (
     "csg_batch_get_field", "csg_batch_nonfinite_step", "csg_batch_device_bytes",
     "csg_schedule_info", "csg_get_schedule", "csg_set_schedule", "csg_autotune",
     "csg_batch_get_schedule", "csg_batch_set_schedule", "csg_batch_autotune", "csg_set_params",
-    "csg_copy_bandwidth", "csg_step_noalloc",
+    "csg_copy_bandwidth", "csg_step_noalloc", "csg_batch_set_persistent",
 )
 
 
@@ -104,6 +104,7 @@ def lib():
         "csg_batch_autotune": (ctypes.c_int, [_VP, dbl, i32]),
         "csg_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(CsgParams)]),
         "csg_copy_bandwidth": (ctypes.c_int, [i32, i64, i32, _D]),
+        "csg_batch_set_persistent": (ctypes.c_int, [_VP, i32]),
         "csg_step_noalloc": (ctypes.c_int, [i32, dbl, ctypes.POINTER(CsgParams), i32, i32, dbl, i64, _VP, _VP, _VP]),
     }
     for name, (res, args) in sig.items():

@@ -31,6 +31,7 @@ UNITS = [
     ("csg_capi.hip", []),
     ("csg_batch.hip", []),
     ("csg_noalloc.hip", []),
+    ("csg_persist.hip", []),
 ] + [(f"csg_ops_{prec}_{group}.hip", []) for prec in ("fp64", "mixed") for group in ("small", "mid", "big", "pow3")]
 
 

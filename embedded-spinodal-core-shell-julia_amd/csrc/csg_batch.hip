@@ -14,8 +14,11 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "../../include/csg.h"
 #include "csg_host.hpp"
+#include "csg_persist.hpp"
 
 using namespace csg;
 
@@ -46,6 +49,7 @@ struct BatchClass {
   long long* ctr = nullptr;      // 2 per instance
   double* hist = nullptr;        // m x hist_cap x 6
   size_t hist_cap = 0;           // rows per instance
+  int pitems[4] = {0, 0, 0, 0};  // work items per member and pass of the persistent kernel (0: no such kernel)
 };
 
 }  // namespace
@@ -66,6 +70,16 @@ struct csg_batch {
   int64_t gevery = 1;   // history interval captured in gexec
   size_t bytes = 0;
   int nt = 0;   // non-temporal pass outputs (nontemporal_for the ensemble's footprint)
+  // persistent mode (csg_batch_set_persistent): one launch per block of steps
+  bool persistent = false;
+  int pG = 0;                              // resident workgroups of the persistent kernel
+  int ninst = 0;                           // members (counters)
+  std::unique_ptr<PersistPlan> hplan;      // host copy of the device plan
+  PersistPlan* dplan = nullptr;
+  long long* dranges = nullptr;
+  unsigned* dctr = nullptr;
+  int* dflag = nullptr;
+  hipGraphExec_t pexec = nullptr;          // captured block of STEPS_PER_GRAPH iterations
 };
 
 namespace {
@@ -219,6 +233,92 @@ int ensure_batch_hist(csg_batch* b, size_t rows) {
   return CSG_OK;
 }
 
+// ---- persistent mode --------------------------------------------------------
+
+// Wall-time limit of one dependency wait of the persistent kernel: far above
+// any step (milliseconds), far below the harness' silence limit.
+constexpr long long PERSIST_WAIT_TICKS = 200000000LL;   // 2 s at 100 MHz
+
+// The plan of one persistent launch: per class its pass arguments (as the
+// batched launches bind them, with the persistent schedules' partial
+// layout) and the flattened item offsets per pass.
+void build_persist_plan(const csg_batch* b, double dt, int64_t every, PersistPlan* pl) {
+  std::memset(static_cast<void*>(pl), 0, sizeof(PersistPlan));
+  pl->ncls = static_cast<int>(b->cls.size());
+  int base = 0;
+  for (int c = 0; c < pl->ncls; ++c) {
+    const BatchClass& k = b->cls[c];
+    PersistClass& pc = pl->cls[c];
+    pc.n = k.n;
+    pc.ninst = k.m;
+    pc.inst_base = base;
+    base += k.m;
+    pc.per_step = 0;
+    for (int q = 0; q < 4; ++q) {
+      pc.items[q] = k.pitems[q];
+      pc.per_step += k.pitems[q];
+    }
+    pc.tw = k.tw;
+    double* p2 = k.part + 5 * k.pitems[0];
+    double* p3 = p2 + k.pitems[1];
+    PassArgs a = class_args(k);
+    bind_in(k, a, k.S, nullptr);
+    bind_out(k, a, k.A, nullptr);
+    a.part = k.part;
+    pc.args[0] = a;
+    a = class_args(k);
+    bind_in(k, a, k.S, k.A);
+    bind_out(k, a, k.A, k.Bb);
+    a.part = p2;
+    pc.args[1] = a;
+    a = class_args(k);
+    bind_in(k, a, k.A, k.Bb);
+    bind_out(k, a, k.A, k.Bb);
+    a.part = p3;
+    pc.args[2] = a;
+    a = class_args(k);
+    bind_in(k, a, k.A, k.Bb);
+    bind_out(k, a, k.S, nullptr);
+    a.dt = dt;
+    FinalArgs f = class_final(k, k.hist, dt, every);
+    f.p2 = p2;
+    f.p3 = p3;
+    f.n1 = k.pitems[0];
+    f.n2 = k.pitems[1];
+    f.n3 = k.pitems[2];
+    a.fin = f;
+    pc.args[3] = a;
+  }
+  for (int q = 0; q < 4; ++q) {
+    pl->off[q][0] = 0;
+    for (int c = 0; c < pl->ncls; ++c)
+      pl->off[q][c + 1] = pl->off[q][c] + static_cast<long long>(pl->cls[c].ninst) * pl->cls[c].items[q];
+  }
+}
+
+// Contiguous item ranges per workgroup and pass, of equal estimated cost
+// (sites per item: an item of an N x N member covers N^2 / items sites).
+std::vector<long long> persist_ranges(const PersistPlan& pl, int G) {
+  std::vector<long long> r(4 * static_cast<size_t>(G + 1), 0);
+  for (int q = 0; q < 4; ++q) {
+    double total = 0.0;
+    for (int c = 0; c < pl.ncls; ++c) total += static_cast<double>(pl.cls[c].ninst) * pl.cls[c].n * pl.cls[c].n;
+    long long* rq = r.data() + static_cast<size_t>(q) * (G + 1);
+    double acc = 0.0;
+    int w = 1;
+    for (int c = 0; c < pl.ncls; ++c) {
+      const PersistClass& k = pl.cls[c];
+      const double cost = static_cast<double>(k.n) * k.n / k.items[q];
+      for (long long g = pl.off[q][c]; g < pl.off[q][c + 1]; ++g) {
+        acc += cost;
+        while (w < G && acc >= total * w / G) rq[w++] = g + 1;
+      }
+    }
+    while (w <= G) rq[w++] = pl.off[q][pl.ncls];
+  }
+  return r;
+}
+
 int check_batch(const csg_batch* b, int32_t i) {
   if (!b) return fail(CSG_ERR_ARG, "null ensemble");
   if (i < 0 || i >= b->count) return fail(CSG_ERR_ARG, "instance index out of range");
@@ -283,6 +383,11 @@ int csg_batch_create(csg_batch** out, int32_t count, const int32_t* n, const dou
     k.nn = static_cast<size_t>(nk) * nk;
     k.tsize = precision == CSG_FP64 ? sizeof(double) : sizeof(float);
     k.part_stride = 5LL * k.ops.max_row1 + k.ops.max_col + k.ops.max_row3;
+    if (persist_size_supported(nk)) {
+      persist_items_of(nk, precision == CSG_FP64, k.pitems);
+      const long long need = 5LL * k.pitems[0] + k.pitems[1] + k.pitems[2];
+      if (need > k.part_stride) k.part_stride = need;
+    }
     if ((e = hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->join[ci], hipEventDisableTiming)) != hipSuccess)
       return cleanup(fail(CSG_ERR_HIP, std::string("stream/event: ") + hipGetErrorString(e)));
@@ -325,6 +430,10 @@ int csg_batch_destroy(csg_batch* b) {
   for (BatchClass& k : b->cls)
     if (k.stream) (void)hipStreamSynchronize(k.stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->pexec) (void)hipGraphExecDestroy(b->pexec);
+  for (void* p : {static_cast<void*>(b->dplan), static_cast<void*>(b->dranges), static_cast<void*>(b->dctr),
+                  static_cast<void*>(b->dflag)})
+    if (p) (void)hipFree(p);
   for (BatchClass& k : b->cls) {
     void* bufs[] = {k.S, k.A, k.Bb, k.phi, k.real, k.part, k.ctr, k.hist, k.dparams, k.tw64};
     for (void* p : bufs)
@@ -388,15 +497,55 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
     HIPCHK(hipStreamSynchronize(b->stream));
   }
   int64_t done = 0;
-  if (steps >= 2 * STEPS_PER_GRAPH) {
-    if (int rc = build_batch_graph(b, dt, hist_every)) return rc;
-    while (steps - done >= STEPS_PER_GRAPH) {
-      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-      done += STEPS_PER_GRAPH;
+  if (b->persistent) {
+    // one launch per block of iterations; the plan (pass arguments, dt,
+    // history binding) is device data, so the captured block stays valid
+    build_persist_plan(b, dt, hist_every, b->hplan.get());
+    HIPCHK(hipMemcpyAsync(b->dplan, b->hplan.get(), sizeof(PersistPlan), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemsetAsync(b->dflag, 0, sizeof(int), b->stream));
+    PersistArgs pa{};
+    pa.plan = b->dplan;
+    pa.ranges = b->dranges;
+    pa.ctr = b->dctr;
+    pa.flag = b->dflag;
+    pa.spin_ticks = PERSIST_WAIT_TICKS;
+    const bool f64 = b->precision == CSG_FP64;
+    if (steps >= 2 * STEPS_PER_GRAPH) {
+      if (!b->pexec) {
+        pa.steps = STEPS_PER_GRAPH;
+        pa.last_advances = 1;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = persist_launch(f64, pa, b->pG, b->ninst, b->stream);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (le != hipSuccess) return fail(CSG_ERR_HIP, std::string("persistent launch: ") + hipGetErrorString(le));
+        if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->pexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          b->pexec = nullptr;
+          return fail(CSG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+      }
+      while (steps - done >= STEPS_PER_GRAPH) {
+        HIPCHK(hipGraphLaunch(b->pexec, b->stream));
+        done += STEPS_PER_GRAPH;
+      }
     }
+    pa.steps = static_cast<int>(steps - done) + 1;
+    pa.last_advances = 0;
+    HIPCHK(persist_launch(f64, pa, b->pG, b->ninst, b->stream));
+  } else {
+    if (steps >= 2 * STEPS_PER_GRAPH) {
+      if (int rc = build_batch_graph(b, dt, hist_every)) return rc;
+      while (steps - done >= STEPS_PER_GRAPH) {
+        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        done += STEPS_PER_GRAPH;
+      }
+    }
+    // the remaining iterations, then the last history row without a step
+    if (int rc = enqueue_iterations(b, dt, static_cast<int>(steps - done) + 1, false, hist_every)) return rc;
   }
-  // the remaining iterations, then the last history row without a step
-  if (int rc = enqueue_iterations(b, dt, static_cast<int>(steps - done) + 1, false, hist_every)) return rc;
   std::vector<std::vector<long long>> ctr(b->cls.size());
   std::vector<std::vector<double>> h(b->cls.size());
   for (size_t ci = 0; ci < b->cls.size(); ++ci) {
@@ -412,7 +561,12 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
       HIPCHK(hipMemcpyAsync(h[ci].data(), k.hist, len * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     }
   }
+  int stalled = 0;
+  if (b->persistent) HIPCHK(hipMemcpyAsync(&stalled, b->dflag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (stalled)
+    return fail(CSG_ERR_HIP, "persistent ensemble launch stalled (a dependency wait exceeded its time limit); "
+                             "the members' state is undefined");
   bool any_bad = false;
   long long first_bad = -1;
   int bad_inst = -1;
@@ -435,6 +589,50 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
   if (any_bad)
     return fail(CSG_ERR_NONFINITE, "Non-finite composition at step " + std::to_string(first_bad) + " (instance " +
                                        std::to_string(bad_inst) + ")");
+  return CSG_OK;
+}
+
+int csg_batch_set_persistent(csg_batch* b, int32_t enable) {
+  if (!b) return fail(CSG_ERR_ARG, "null ensemble");
+  if (!enable) {
+    b->persistent = false;
+    return CSG_OK;
+  }
+  if (b->persistent) return CSG_OK;
+  if (b->cls.size() > static_cast<size_t>(PERSIST_MAX_CLASSES))
+    return fail(CSG_ERR_UNSUPPORTED, "the persistent kernel takes at most 4 grid sizes");
+  for (const BatchClass& k : b->cls)
+    if (!k.pitems[0])
+      return fail(CSG_ERR_UNSUPPORTED, "no persistent kernel for grid size " + std::to_string(k.n) +
+                                           " (compiled: 128, 256, 384, 512)");
+  HIPCHK(hipSetDevice(b->device));
+  int per_cu = 0, dev = 0;
+  if (persist_blocks_per_cu(b->precision == CSG_FP64, &per_cu) || per_cu < 1)
+    return fail(CSG_ERR_HIP, "persistent kernel: occupancy query failed");
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  b->pG = per_cu * prop.multiProcessorCount;
+  b->ninst = 0;
+  for (const BatchClass& k : b->cls) b->ninst += k.m;
+  if (!b->hplan) b->hplan.reset(new PersistPlan());
+  build_persist_plan(b, 0.0, 1, b->hplan.get());
+  const std::vector<long long> ranges = persist_ranges(*b->hplan, b->pG);
+  if (!b->dplan && bmalloc(b, reinterpret_cast<void**>(&b->dplan), sizeof(PersistPlan))) return CSG_ERR_HIP;
+  if (b->dranges) {
+    HIPCHK(hipFree(b->dranges));
+    b->dranges = nullptr;
+  }
+  if (bmalloc(b, reinterpret_cast<void**>(&b->dranges), ranges.size() * sizeof(long long))) return CSG_ERR_HIP;
+  if (!b->dctr && bmalloc(b, reinterpret_cast<void**>(&b->dctr), static_cast<size_t>(b->ninst) * sizeof(unsigned)))
+    return CSG_ERR_HIP;
+  if (!b->dflag && bmalloc(b, reinterpret_cast<void**>(&b->dflag), sizeof(int))) return CSG_ERR_HIP;
+  HIPCHK(hipMemcpy(b->dranges, ranges.data(), ranges.size() * sizeof(long long), hipMemcpyHostToDevice));
+  if (b->pexec) {
+    HIPCHK(hipGraphExecDestroy(b->pexec));
+    b->pexec = nullptr;
+  }
+  b->persistent = true;
   return CSG_OK;
 }
 

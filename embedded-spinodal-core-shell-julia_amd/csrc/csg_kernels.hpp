@@ -117,9 +117,8 @@ __device__ __forceinline__ FinalArgs final_view(const FinalArgs& f0, int q) {
   return f;
 }
 
-// PassArgs of the instance this workgroup works on (blockIdx.y).
-__device__ __forceinline__ PassArgs instance_view(const PassArgs& a0) {
-  const int q = blockIdx.y;
+// PassArgs of instance q (the grid row of a batched launch).
+__device__ __forceinline__ PassArgs instance_view(const PassArgs& a0, int q) {
   if (q == 0 && !a0.is.params) return a0;
   PassArgs a = a0;
   a.in0 = offset_bytes(a0.in0, q, a0.is.in0);
@@ -276,6 +275,7 @@ struct Sched {
 
 template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
 struct RowCfg {
+  static constexpr int NN = N;
   static constexpr int L = N / 2;
   static constexpr int E = SC::E ? SC::E : fft_E(L);
   static constexpr int TPL = L / E;
@@ -298,24 +298,41 @@ struct RowCfg {
   static_assert(NT <= 1024 && (NT % 64 == 0 || (NT < 64 && (NT & (NT - 1)) == 0)), "workgroup of whole wavefronts");
 };
 
-template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
-__global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE, SC>::NT))
-row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
+// LDS of one row-pass workgroup, carved from one byte array: the tile and
+// exchange buffers, the twiddle half table, the stage tables and the
+// reduction scratch.
+template <class Cfg, class CT>
+struct RowShared {
+  static constexpr int NLDS = Cfg::LDS, NTW = tw_in_lds<Cfg::NN, CT>() ? Cfg::NN / 2 : 0,
+                       NST = Cfg::ST ? stab_len(Cfg::L, Cfg::E) : 0, NSCR = 8 * ((Cfg::NT + 63) / 64);
+  static constexpr int OFF_TW = NLDS * (int)sizeof(cplx<CT>);
+  static constexpr int OFF_ST = OFF_TW + NTW * (int)sizeof(cplx<CT>);
+  static constexpr int OFF_SCR = OFF_ST + NST * (int)sizeof(cplx<CT>);
+  static constexpr int BYTES = OFF_SCR + NSCR * (int)sizeof(double);
+};
+
+// Body of a row pass for tile-workgroup `bx` of instance q, with LDS at
+// `smem` (RowShared bytes): the batched kernel below runs it for
+// (blockIdx.x, blockIdx.y); a persistent kernel for its work items.
+template <class CT, class TIN, class TOUT, int N, int MODE, class SC>
+__device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, const cplx<CT>* __restrict__ tw_g,
+                                         char* smem) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE, SC>;
-  const PassArgs a = instance_view(a0);
+  using Sh = RowShared<Cfg, CT>;
+  const PassArgs a = instance_view(a0, q);
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr int NARR = Cfg::NARR;
   constexpr bool WAVE = Cfg::WAVE;
   constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
   constexpr bool ST = Cfg::ST;
-  __shared__ cplx<CT> lds[Cfg::LDS];
-  __shared__ cplx<CT> tw_s[tw_lds_len<N, CT>()];
-  __shared__ cplx<CT> st_s[ST ? stab_len(L, E) : 1];
-  __shared__ double scratch[8 * ((NT + 63) / 64)];
+  cplx<CT>* lds = reinterpret_cast<cplx<CT>*>(smem);
+  cplx<CT>* tw_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_TW);
+  cplx<CT>* st_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_ST);
+  double* scratch = reinterpret_cast<double*>(smem + Sh::OFF_SCR);
   const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;   // real-FFT pre/post twiddles
   const cplx<CT>* ftw = ST ? st_s : tw;                     // FFT engine twiddles
   // split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2 TILES) take Gy/Jy
-  const int tile = xcd_tile(blockIdx.x, Cfg::GRID);
+  const int tile = xcd_tile(bx, Cfg::GRID);
   const int arr0 = (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0;
   const int j0 = (tile - arr0 * Cfg::TILES) * B;
   const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -538,7 +555,7 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     const int ops[5] = {RED_SUM, RED_SUM, RED_MIN, RED_MAX, RED_SUM};
     block_reduce<NT, 5>(red, ops, scratch);
     if (threadIdx.x == 0) {
-      double* rec = a.part + 5 * blockIdx.x;
+      double* rec = a.part + 5 * bx;
 #pragma unroll
       for (int q = 0; q < 5; ++q) rec[q] = red[q];
     }
@@ -547,8 +564,16 @@ row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
     double r[1] = {red[0]};
     const int ops[1] = {RED_SUM};
     block_reduce<NT, 1>(r, ops, scratch);
-    if (threadIdx.x == 0) a.part[blockIdx.x] = r[0];
+    if (threadIdx.x == 0) a.part[bx] = r[0];
   }
+}
+
+template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
+__global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE, SC>::NT))
+row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
+  using Cfg = RowCfg<CT, TIN, TOUT, N, MODE, SC>;
+  __shared__ __attribute__((aligned(16))) char smem[RowShared<Cfg, CT>::BYTES];
+  row_body<CT, TIN, TOUT, N, MODE, SC>(a0, blockIdx.x, blockIdx.y, tw_g, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +592,7 @@ constexpr int col_lines() {
 
 template <class CT, class TS, int N, int MODE, class SC = Sched<>>
 struct ColCfg {
+  static constexpr int NN = N;
   static constexpr int E = SC::E ? SC::E : fft_E(N);
   static constexpr int TPL = N / E;
   static constexpr int C = SC::W ? SC::W : col_lines<N, CT, E>();
@@ -589,28 +615,42 @@ struct ColCfg {
   static_assert(NT <= 1024 && (NT % 64 == 0 || (NT < 64 && (NT & (NT - 1)) == 0)), "workgroup of whole wavefronts");
 };
 
-template <class CT, class TS, int N, int MODE, class SC = Sched<>>
-__global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE, SC>::NT))
-col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
+// LDS of one column-pass workgroup: line exchange buffers, the stage tables
+// or the half table, the reduction scratch.
+template <class Cfg, class CT>
+struct ColShared {
+  static constexpr int NLDS = Cfg::LDS,
+                       NTW = Cfg::ST ? stab_len(Cfg::NN, Cfg::E) : (tw_in_lds<Cfg::NN, CT>() ? Cfg::NN / 2 : 0),
+                       NSCR = 8 * ((Cfg::NT + 63) / 64);
+  static constexpr int OFF_TW = NLDS * (int)sizeof(cplx<CT>);
+  static constexpr int OFF_SCR = OFF_TW + NTW * (int)sizeof(cplx<CT>);
+  static constexpr int BYTES = OFF_SCR + NSCR * (int)sizeof(double);
+};
+
+// Body of a column pass for workgroup `bx` of instance q (P4: bx = 0 is the
+// history-row reduction, bx - 1 the line group), with LDS at `smem`.
+template <class CT, class TS, int N, int MODE, class SC>
+__device__ __forceinline__ void col_body(const PassArgs& a0, int bx, int q, const cplx<CT>* __restrict__ tw_g,
+                                         char* smem) {
   using Cfg = ColCfg<CT, TS, N, MODE, SC>;
-  const PassArgs a = instance_view(a0);
+  using Sh = ColShared<Cfg, CT>;
+  const PassArgs a = instance_view(a0, q);
   constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
   constexpr bool WAVE = Cfg::WAVE;
   constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
   constexpr bool ST = Cfg::ST;
-  __shared__ cplx<CT> lds[Cfg::LDS];
-  // the stage tables, or the half table
-  __shared__ cplx<CT> tw_s[ST ? stab_len(N, E) : tw_lds_len<N, CT>()];
-  __shared__ double scratch[8 * ((NT + 63) / 64)];
+  cplx<CT>* lds = reinterpret_cast<cplx<CT>*>(smem);
+  cplx<CT>* tw_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_TW);   // the stage tables, or the half table
+  double* scratch = reinterpret_cast<double*>(smem + Sh::OFF_SCR);
   const cplx<CT>* tw = (ST || tw_in_lds<N, CT>()) ? tw_s : tw_g;
   if constexpr (MODE == COL_P4) {
     // the partials of P1..P3 of this step are complete: history row
-    if (blockIdx.x == 0) {
+    if (bx == 0) {
       if (a.fin.hist != nullptr) finalize_block<NT>(a.fin, scratch);
       return;
     }
   }
-  const int lb = MODE == COL_P4 ? static_cast<int>(blockIdx.x) - 1 : static_cast<int>(blockIdx.x);
+  const int lb = MODE == COL_P4 ? bx - 1 : bx;
   const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const int kx = lb * C + l;
   cplx<CT>* line = lds + l * LP;
@@ -710,7 +750,7 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       double r[1] = {esum};
       const int ops[1] = {RED_SUM};
       block_reduce<NT, 1>(r, ops, scratch);
-      if (threadIdx.x == 0) a.part[blockIdx.x] = r[0];
+      if (threadIdx.x == 0) a.part[bx] = r[0];
     } else {
       cplx<CT> w[1][E];
 #pragma unroll
@@ -803,6 +843,14 @@ col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
       }
     }
   }
+}
+
+template <class CT, class TS, int N, int MODE, class SC = Sched<>>
+__global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE, SC>::NT))
+col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
+  using Cfg = ColCfg<CT, TS, N, MODE, SC>;
+  __shared__ __attribute__((aligned(16))) char smem[ColShared<Cfg, CT>::BYTES];
+  col_body<CT, TS, N, MODE, SC>(a0, blockIdx.x, blockIdx.y, tw_g, smem);
 }
 
 // ---------------------------------------------------------------------------

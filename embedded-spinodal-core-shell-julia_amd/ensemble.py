@@ -69,7 +69,8 @@ class Ensemble:
     """Device state of an ensemble: one ``csg_batch`` handle."""
 
     def __init__(self, ns: Sequence[int], spacings: Optional[Sequence[float]] = None,
-                 params: Optional[Sequence[Parameters]] = None, precision: str = "mixed", device: int = 0):
+                 params: Optional[Sequence[Parameters]] = None, precision: str = "mixed", device: int = 0,
+                 persistent: bool = False):
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {sorted(_PREC)}")
         self.ns = [int(n) for n in ns]
@@ -92,7 +93,16 @@ class Ensemble:
         check(lib().csg_batch_create(h, k, n_arr, sp, pa, _PREC[precision], int(device)))
         self.handle = h
         self.precision = precision
+        self.persistent = False
         self._fin = weakref.finalize(self, lib().csg_batch_destroy, h)
+        if persistent:
+            self.set_persistent(True)
+
+    def set_persistent(self, enable: bool = True):
+        """Run every pass of every member in one persistent launch per block
+        of steps (sizes 128, 256, 384, 512) instead of batched launches."""
+        check(lib().csg_batch_set_persistent(self.handle, 1 if enable else 0))
+        self.persistent = bool(enable)
 
     def __len__(self):
         return len(self.ns)
@@ -149,11 +159,11 @@ class Ensemble:
 
 def run_ensemble(c0s: Sequence, phis: Sequence, dt: float = 0.01, steps: int = 1000,
                  spacings: Optional[Sequence[float]] = None, params: Optional[Sequence[Parameters]] = None, *,
-                 precision: str = "mixed", history_every: int = 1, device: int = 0):
+                 precision: str = "mixed", history_every: int = 1, device: int = 0, persistent: bool = False):
     """``run_simulation`` of every member at once: returns [(c, history), ...]."""
     if len(c0s) != len(phis):
         raise ValueError("one phi per c0")
-    ens = Ensemble([np.shape(c)[0] for c in c0s], spacings, params, precision, device)
+    ens = Ensemble([np.shape(c)[0] for c in c0s], spacings, params, precision, device, persistent)
     try:
         for i, (c0, phi) in enumerate(zip(c0s, phis)):
             ens.set_fields(i, c0, phi)

@@ -212,6 +212,17 @@ int csg_batch_run(csg_batch* b, double dt, int64_t steps, int64_t hist_every, do
 /* Download instance i's composition field (column-major n[i] x n[i]). */
 int csg_batch_get_field(csg_batch* b, int32_t i, double* c_out);
 
+/* Persistent mode (enable = 1): csg_batch_run executes every pass of every
+ * member for a block of 32 loop iterations in ONE launch of a persistent
+ * kernel (captured in a graph), with per-member completion counters in place
+ * of kernel boundaries, instead of one batched launch per size class and
+ * pass.  Grid sizes 128, 256, 384 and 512 only (CSG_ERR_UNSUPPORTED
+ * otherwise), at most 4 of them; the kernel uses its own fixed schedules (the
+ * members' csg_batch_set_schedule choices apply to the batched mode only), so
+ * results agree with the batched mode to rounding.  enable = 0 returns to the
+ * batched mode. */
+int csg_batch_set_persistent(csg_batch* b, int32_t enable);
+
 /* First non-finite step of instance i in the last csg_batch_run, or -1. */
 int64_t csg_batch_nonfinite_step(const csg_batch* b, int32_t i);
 

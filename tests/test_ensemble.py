@@ -210,3 +210,61 @@ def test_ensemble_c3_full_workload_properties(pkg):
         c = ens.field(i)
         assert np.all(np.isfinite(c)) and c.shape == (ms[i].n, ms[i].n)
     ens.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_persistent_ensemble_matches_oracle_and_batched(pkg, precision):
+    """The persistent kernel (every pass of every member in one launch per
+    32 steps, per-member counters) on all four C3 sizes: FP64 members against
+    the oracle to 1e-12 after 70 steps (two captured blocks plus an eager
+    tail), both precisions against the batched mode, mean composition kept."""
+    ms = _c3_members(pkg)
+    steps = 70
+    say(f"persistent ensemble {precision}, {steps} steps, sizes {[m.n for m in ms]}")
+    fields = [pkg.member_fields(m) for m in ms]
+    out = {}
+    for persistent in (False, True):
+        out[persistent] = pkg.run_ensemble([f[0] for f in fields], [f[1] for f in fields], steps=steps,
+                                           params=[m.p for m in ms], precision=precision,
+                                           persistent=persistent)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for i, m in enumerate(ms):
+        (cp, hp), (cb, hb) = out[True][i], out[False][i]
+        assert hp.shape == hb.shape == (steps + 1, 6)
+        assert np.max(np.abs(cp - cb)) < tol, (i, m.n)
+        assert np.max(np.abs(hp[:, 2] - hb[:, 2]) / np.abs(hb[:, 2])) < tol, (i, m.n)
+        assert np.max(np.abs(hp[:, 1] - hp[0, 1])) <= 1e-13, (i, m.n)
+        if precision == "fp64":
+            po = o.Parameters(**dataclasses.asdict(m.p))
+            cr, hr = o.run_simulation_fast(np.asarray(fields[i][0]), np.asarray(fields[i][1]), steps=steps, p=po)
+            assert np.max(np.abs(cp - cr)) < 1e-12, (i, m.n)
+            assert np.max(np.abs(hp - hr) / np.maximum(1.0, np.abs(hr))) < 1e-12, (i, m.n)
+
+
+@pytest.mark.gpu
+def test_persistent_ensemble_history_every_and_second_run(pkg):
+    """history_every and a second run continuing from the resident fields,
+    persistent mode against batched mode (FP64)."""
+    ms = _c3_members(pkg)[::2]
+    say("persistent ensemble history_every=7, two runs")
+    fields = [pkg.member_fields(m) for m in ms]
+    res = []
+    for persistent in (False, True):
+        ens = pkg.Ensemble([m.n for m in ms], params=[m.p for m in ms], precision="fp64", persistent=persistent)
+        for i, (c0, phi) in enumerate(fields):
+            ens.set_fields(i, c0, phi)
+        h1 = ens.run(0.01, 40, history_every=7)
+        h2 = ens.run(0.01, 5)
+        res.append((h1, h2, [ens.field(i) for i in range(len(ms))]))
+        ens.close()
+    for i in range(len(ms)):
+        assert res[1][0][i].shape == (6, 6)
+        assert np.max(np.abs(res[1][0][i] - res[0][0][i]) / np.maximum(1.0, np.abs(res[0][0][i]))) < 1e-12
+        assert np.max(np.abs(res[1][1][i] - res[0][1][i]) / np.maximum(1.0, np.abs(res[0][1][i]))) < 1e-12
+        assert np.max(np.abs(res[1][2][i] - res[0][2][i])) < 1e-12
+
+
+def test_persistent_mode_rejects_other_sizes_without_gpu(pkg):
+    from csg_amd import _capi
+    assert pkg.lib().csg_batch_set_persistent(None, 1) == _capi.CSG_ERR_ARG
