@@ -311,35 +311,30 @@ struct RowShared {
   static constexpr int BYTES = OFF_SCR + NSCR * (int)sizeof(double);
 };
 
-// Body of a row pass for tile-workgroup `bx` of instance q, with LDS at
-// `smem` (RowShared bytes): the batched kernel below runs it for
-// (blockIdx.x, blockIdx.y); a persistent kernel for its work items.
+// Body of a row pass for tile-workgroup `bx` of instance q, with its LDS
+// regions (tile and exchange buffers, twiddle half table, stage tables,
+// reduction scratch; RowShared sizes): the batched kernel below runs it for
+// (blockIdx.x, blockIdx.y) on separate __shared__ arrays (distinct objects,
+// so the compiler knows they do not alias), a persistent kernel for its work
+// items on regions of one buffer.
 template <class CT, class TIN, class TOUT, int N, int MODE, class SC>
 __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, const cplx<CT>* __restrict__ tw_g,
-                                         char* smem) {
+                                         cplx<CT>* lds, cplx<CT>* tw_s, cplx<CT>* st_s, double* scratch) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE, SC>;
-  using Sh = RowShared<Cfg, CT>;
   const PassArgs a = instance_view(a0, q);
   constexpr int L = Cfg::L, E = Cfg::E, TPL = Cfg::TPL, B = Cfg::B, LP = Cfg::LP, NT = Cfg::NT;
   constexpr int NARR = Cfg::NARR;
   constexpr bool WAVE = Cfg::WAVE;
   constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
   constexpr bool ST = Cfg::ST;
-  cplx<CT>* lds = reinterpret_cast<cplx<CT>*>(smem);
-  cplx<CT>* tw_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_TW);
-  cplx<CT>* st_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_ST);
-  double* scratch = reinterpret_cast<double*>(smem + Sh::OFF_SCR);
   const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;   // real-FFT pre/post twiddles
   const cplx<CT>* ftw = ST ? st_s : tw;                     // FFT engine twiddles
-  // split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2 TILES) take Gy/Jy
-  const int tile = xcd_tile(bx, Cfg::GRID);
-  const int arr0 = (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0;
-  const int j0 = (tile - arr0 * Cfg::TILES) * B;
   const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
   cplx<CT>* row = lds + b * LP;                 // FFT exchange buffer of this row
-  const size_t rbase = (size_t)(j0 + b) * N;   // real-row offset
   cplx<CT> v[NARR][E];
-  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // partials, reduced at the end
+  // partials, reduced at the end: P1 [sum f, sum of row sums, min c, max c,
+  // #non-finite]; P3 [dissipation]
+  double red[5] = {0.0, 0.0, INFINITY, -INFINITY, 0.0};
 
   // Loads are issued before the first wait where the schedule says so (tile
   // group, phi, twiddles): one memory latency instead of one per phase.
@@ -347,8 +342,11 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   constexpr int G = Cfg::G;
   constexpr bool PF = Cfg::PF != 0 && (MODE == ROW_P1 || MODE == ROW_P3);
   constexpr bool PHI_HINT = std::is_same<CT, double>::value && N >= PHI_HINT_MIN_N;
-  CT ph[E][2];   // phi at this thread's sites (P1, P3)
-  auto load_phi = [&]() {
+  // tile -> component (split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2
+  // TILES) take Gy/Jy) and first row
+  auto arr_of = [&](int tile) { return (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0; };
+  auto row0_of = [&](int tile) { return (tile - arr_of(tile) * Cfg::TILES) * B; };
+  auto load_phi = [&](size_t rbase, CT (&ph)[E][2]) {
     const CT* phi = static_cast<const CT*>(a.phi);
     // The NT_PHI branch is compiled only where the hint can be set (FP64
     // fields of 4096^2 and up); elsewhere it would cost its control flow.
@@ -371,51 +369,35 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
       ph[j][1] = phi[rbase + 2 * k + 1];
     }
   };
-  if constexpr (MODE == ROW_FWD) {
-    const double* x = static_cast<const double*>(a.in0);
+  // transposed tile loads from line layout: lane-consecutive rows (coalesced)
+  auto tile_load = [&](int arr0, int j0, int g0, cplx<TIN>(&tl)[NARR][E]) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int k = t + TPL * j;
-      v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
-    }
-    if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
-    if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
-    __syncthreads();
-  } else {
-    // transposed tile load from line layout: lane-consecutive rows (coalesced),
-    // in groups of G elements per thread
+    for (int q = 0; q < NARR; ++q) {
+      const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
 #pragma unroll
-    for (int g0 = 0; g0 < E; g0 += G) {
-      cplx<TIN> tl[NARR][G];
-#pragma unroll
-      for (int q = 0; q < NARR; ++q) {
-        const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
-#pragma unroll
-        for (int r = 0; r < G; ++r) {
-          const int f = threadIdx.x + NT * (g0 + r);
-          tl[q][r] = X[(size_t)(f / B) * N + j0 + f % B];
-        }
+      for (int r = 0; r < G; ++r) {
+        const int f = threadIdx.x + NT * (g0 + r);
+        tl[q][g0 + r] = X[(size_t)(f / B) * N + j0 + f % B];
       }
-      if (g0 == 0) {
-        if constexpr (MODE == ROW_P1) {
-          // mean composition from the FP64 k_x = 0 row sums of the state
-          if (threadIdx.x < B) red[1] = static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x);
-        }
-        if constexpr (PF) load_phi();
-        if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
-        if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
-      }
-#pragma unroll
-      for (int q = 0; q < NARR; ++q)
-#pragma unroll
-        for (int r = 0; r < G; ++r) {
-          const int f = threadIdx.x + NT * (g0 + r);
-          lds[(q * B + f % B) * LP + pad<CT, L>(f / B)] = cvt<CT>(tl[q][r]);
-        }
     }
-    __syncthreads();
-    // C2R pre-processing fused with the read: Z_k = (X_k + conj X_{L-k}) +
-    // i (X_k - conj X_{L-k}) W_N^{-k}, times 1/N; slot 0 holds (X_0, X_L).
+  };
+  auto tile_to_lds = [&](int g0, const cplx<TIN>(&tl)[NARR][E]) {
+#pragma unroll
+    for (int q = 0; q < NARR; ++q)
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        const int f = threadIdx.x + NT * (g0 + r);
+        lds[(q * B + f % B) * LP + pad<CT, L>(f / B)] = cvt<CT>(tl[q][g0 + r]);
+      }
+  };
+  // mean composition from the FP64 k_x = 0 row sums of the state (P1)
+  auto row_sum = [&](int j0) {
+    return threadIdx.x < B ? static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x) : 0.0;
+  };
+  // C2R pre-processing fused with the read: Z_k = (X_k + conj X_{L-k}) +
+  // i (X_k - conj X_{L-k}) W_N^{-k}, times 1/N; slot 0 holds (X_0, X_L).
+  // Then the inverse FFT along i.
+  auto c2r = [&]() {
     const CT scale = static_cast<CT>(1.0 / N);
 #pragma unroll
     for (int q = 0; q < NARR; ++q) {
@@ -434,121 +416,158 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
     }
     lds_sync<WAVE>();
     fft_regs<CT, +1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
-  }
-
-  if constexpr (MODE == ROW_INV) {
-    double* x = static_cast<double*>(a.out0);
+  };
+  // pointwise work at this thread's sites (P1: f'(c); bulk energy, min,
+  // max, non-finite partials.  P3: J = M0 h(phi) g; dissipation partial),
+  // then R2C along i (forward FFT, X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k}))
+  // and the transposed store of the tile to line layout
+  auto pointwise_r2c_store = [&](int arr0, int j0, const CT(&ph)[E][2]) {
+    if constexpr (MODE == ROW_P1) {
+      const DevParams& p = a.p;
+      const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
+      double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int k = t + TPL * j;
-      x[rbase + 2 * k] = static_cast<double>(v[0][j].x);
-      x[rbase + 2 * k + 1] = static_cast<double>(v[0][j].y);
-    }
-    return;
-  }
-
-  if constexpr ((MODE == ROW_P1 || MODE == ROW_P3) && !PF) load_phi();
-  if constexpr (MODE == ROW_P1) {
-    // pointwise: f'(c); bulk energy, min, max, non-finite partials
-    const DevParams& p = a.p;
-    const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
-    double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
+      for (int j = 0; j < E; ++j) {
+        const CT cin[2] = {v[0][j].x, v[0][j].y};
+        const CT qq[2] = {ph[j][0], ph[j][1]};
+        CT out[2];
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const CT cin[2] = {v[0][j].x, v[0][j].y};
-      const CT qq[2] = {ph[j][0], ph[j][1]};
-      CT out[2];
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const CT c = cin[e], q = qq[e];
-        const CT h = q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q);   // core_shell.jl:40
-        const CT g = q * q * (CT(1) - q) * (CT(1) - q);                     // :41
-        // bulk_derivative, core_shell.jl:53-54
-        out[e] = CT(2) * A * (CT(1) - h) * (c - ca) +
-                 CT(2) * Bc * h * (c - cb) * (c - cc) * (CT(2) * c - cb - cc) - chi * P * g;
-        // bulk_energy, core_shell.jl:47-48
-        const CT fb = A * (CT(1) - h) * (c - ca) * (c - ca) +
-                      Bc * h * (c - cb) * (c - cb) * (c - cc) * (c - cc) + P * (CT(1) - chi * c) * g;
-        fsum += static_cast<double>(fb);
-        // branchless: non-finite values are counted and kept out of min/max
-        const bool ok = finite(c);
-        const double cd = static_cast<double>(c);
-        cmin = fmin(cmin, ok ? cd : INFINITY);
-        cmax = fmax(cmax, ok ? cd : -INFINITY);
-        nbad += ok ? 0.0 : 1.0;
+        for (int e = 0; e < 2; ++e) {
+          const CT c = cin[e], q = qq[e];
+          const CT h = q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q);   // core_shell.jl:40
+          const CT g = q * q * (CT(1) - q) * (CT(1) - q);                     // :41
+          // bulk_derivative, core_shell.jl:53-54
+          out[e] = CT(2) * A * (CT(1) - h) * (c - ca) +
+                   CT(2) * Bc * h * (c - cb) * (c - cc) * (CT(2) * c - cb - cc) - chi * P * g;
+          // bulk_energy, core_shell.jl:47-48
+          const CT fb = A * (CT(1) - h) * (c - ca) * (c - ca) +
+                        Bc * h * (c - cb) * (c - cb) * (c - cc) * (c - cc) + P * (CT(1) - chi * c) * g;
+          fsum += static_cast<double>(fb);
+          // branchless: non-finite values are counted and kept out of min/max
+          const bool ok = finite(c);
+          const double cd = static_cast<double>(c);
+          cmin = fmin(cmin, ok ? cd : INFINITY);
+          cmax = fmax(cmax, ok ? cd : -INFINITY);
+          nbad += ok ? 0.0 : 1.0;
+        }
+        v[0][j] = mk(out[0], out[1]);
       }
-      v[0][j] = mk(out[0], out[1]);
+      red[0] += fsum;
+      red[2] = fmin(red[2], cmin);
+      red[3] = fmax(red[3], cmax);
+      red[4] += nbad;
     }
-    red[0] = fsum;
-    red[2] = cmin;
-    red[3] = cmax;
-    red[4] = nbad;
-  }
-
-  if constexpr (MODE == ROW_P3) {
-    // J = M0 h(phi) g for the gradient component(s); dissipation partial
-    const CT mob = a.p.mobility;
-    double dsum = 0.0;
+    if constexpr (MODE == ROW_P3) {
+      const CT mob = a.p.mobility;
+      double dsum = 0.0;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const CT q0 = ph[j][0], q1 = ph[j][1];
-      const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
-      const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
+      for (int j = 0; j < E; ++j) {
+        const CT q0 = ph[j][0], q1 = ph[j][1];
+        const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
+        const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
 #pragma unroll
-      for (int q = 0; q < NARR; ++q) {
-        const cplx<CT> g = v[q][j];
-        // core_shell.jl:68 sum(M .* (gx.^2 .+ gy.^2))
-        dsum += static_cast<double>(M0 * (g.x * g.x)) + static_cast<double>(M1 * (g.y * g.y));
-        v[q][j] = mk(M0 * g.x, M1 * g.y);
+        for (int q = 0; q < NARR; ++q) {
+          const cplx<CT> g = v[q][j];
+          // core_shell.jl:68 sum(M .* (gx.^2 .+ gy.^2))
+          dsum += static_cast<double>(M0 * (g.x * g.x)) + static_cast<double>(M1 * (g.y * g.y));
+          v[q][j] = mk(M0 * g.x, M1 * g.y);
+        }
       }
+      red[0] += dsum;
     }
-    red[0] = dsum;
-  }
-
-  // R2C along i: forward FFT, then X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k})
-  fft_regs<CT, -1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
+    fft_regs<CT, -1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
 #pragma unroll
-  for (int q = 0; q < NARR; ++q) {
-    cplx<CT>* tr = lds + (q * B + b) * LP;
+    for (int q = 0; q < NARR; ++q) {
+      cplx<CT>* tr = lds + (q * B + b) * LP;
 #pragma unroll
-    for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
-  }
-  lds_sync<WAVE>();
-  const CT half = static_cast<CT>(0.5);
+      for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
+    }
+    lds_sync<WAVE>();
+    const CT half = static_cast<CT>(0.5);
 #pragma unroll
-  for (int q = 0; q < NARR; ++q) {
-    const cplx<CT>* tr = lds + (q * B + b) * LP;
+    for (int q = 0; q < NARR; ++q) {
+      const cplx<CT>* tr = lds + (q * B + b) * LP;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int k = t + TPL * j;
-      const cplx<CT> zk = v[q][j];
-      if (k == 0) {
-        v[q][j] = mk(zk.x + zk.y, zk.x - zk.y);
-      } else {
-        const cplx<CT> zm = tr[pad<CT, L>(L - k)];
-        const cplx<CT> ek = half * (zk + conj(zm));
-        const cplx<CT> ok = half * mul_mi(zk - conj(zm));
-        v[q][j] = ek + cmul(tw[k], ok);
+      for (int j = 0; j < E; ++j) {
+        const int k = t + TPL * j;
+        const cplx<CT> zk = v[q][j];
+        if (k == 0) {
+          v[q][j] = mk(zk.x + zk.y, zk.x - zk.y);
+        } else {
+          const cplx<CT> zm = tr[pad<CT, L>(L - k)];
+          const cplx<CT> ek = half * (zk + conj(zm));
+          const cplx<CT> ok = half * mul_mi(zk - conj(zm));
+          v[q][j] = ek + cmul(tw[k], ok);
+        }
       }
     }
-  }
-  lds_sync<WAVE>();
+    lds_sync<WAVE>();
 #pragma unroll
-  for (int q = 0; q < NARR; ++q) {
-    cplx<CT>* tr = lds + (q * B + b) * LP;
+    for (int q = 0; q < NARR; ++q) {
+      cplx<CT>* tr = lds + (q * B + b) * LP;
 #pragma unroll
-    for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NARR; ++q) {
-    cplx<TOUT>* X = static_cast<cplx<TOUT>*>((arr0 + q) == 0 ? a.out0 : a.out1);
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      const int f = threadIdx.x + NT * r;
-      const int bb = f % B, kx = f / B;
-      store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT, L>(kx)]), a.nt);
+      for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NARR; ++q) {
+      cplx<TOUT>* X = static_cast<cplx<TOUT>*>((arr0 + q) == 0 ? a.out0 : a.out1);
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const int f = threadIdx.x + NT * r;
+        const int bb = f % B, kx = f / B;
+        store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT, L>(kx)]), a.nt);
+      }
+    }
+  };
+
+  {
+    const int tile = xcd_tile(bx, Cfg::GRID);
+    const int arr0 = arr_of(tile);
+    const int j0 = row0_of(tile);
+    const size_t rbase = (size_t)(j0 + b) * N;   // real-row offset
+    CT ph[E][2];   // phi at this thread's sites (P1, P3)
+    if constexpr (MODE == ROW_FWD) {
+      const double* x = static_cast<const double*>(a.in0);
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int k = t + TPL * j;
+        v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
+      }
+      if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+      if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
+      __syncthreads();
+    } else {
+      // in groups of G elements per thread, each written to LDS before the
+      // next is issued
+      cplx<TIN> tl[NARR][E];
+#pragma unroll
+      for (int g0 = 0; g0 < E; g0 += G) {
+        tile_load(arr0, j0, g0, tl);
+        if (g0 == 0) {
+          if constexpr (MODE == ROW_P1) red[1] = row_sum(j0);
+          if constexpr (PF) load_phi(rbase, ph);
+          if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+          if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
+        }
+        tile_to_lds(g0, tl);
+      }
+      __syncthreads();
+      c2r();
+    }
+
+    if constexpr (MODE == ROW_INV) {
+      double* x = static_cast<double*>(a.out0);
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int k = t + TPL * j;
+        x[rbase + 2 * k] = static_cast<double>(v[0][j].x);
+        x[rbase + 2 * k + 1] = static_cast<double>(v[0][j].y);
+      }
+      return;
+    }
+    if constexpr ((MODE == ROW_P1 || MODE == ROW_P3) && !PF) load_phi(rbase, ph);
+    pointwise_r2c_store(arr0, j0, ph);
   }
 
   if constexpr (MODE == ROW_P1) {
@@ -572,8 +591,12 @@ template <class CT, class TIN, class TOUT, int N, int MODE, class SC = Sched<>>
 __global__ void __launch_bounds__((RowCfg<CT, TIN, TOUT, N, MODE, SC>::NT))
 row_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
   using Cfg = RowCfg<CT, TIN, TOUT, N, MODE, SC>;
-  __shared__ __attribute__((aligned(16))) char smem[RowShared<Cfg, CT>::BYTES];
-  row_body<CT, TIN, TOUT, N, MODE, SC>(a0, blockIdx.x, blockIdx.y, tw_g, smem);
+  using Sh = RowShared<Cfg, CT>;
+  __shared__ cplx<CT> lds[Sh::NLDS];
+  __shared__ cplx<CT> tw_s[Sh::NTW > 0 ? Sh::NTW : 1];
+  __shared__ cplx<CT> st_s[Sh::NST > 0 ? Sh::NST : 1];
+  __shared__ double scratch[Sh::NSCR];
+  row_body<CT, TIN, TOUT, N, MODE, SC>(a0, blockIdx.x, blockIdx.y, tw_g, lds, tw_s, st_s, scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,229 +651,27 @@ struct ColShared {
 };
 
 // Body of a column pass for workgroup `bx` of instance q (P4: bx = 0 is the
-// history-row reduction, bx - 1 the line group), with LDS at `smem`.
+// history-row reduction, bx - 1 the line group), with its LDS regions (line
+// buffers, stage tables or half table, reduction scratch; ColShared sizes),
+// for the persistent kernel.  The batched kernel col_pass includes the same
+// statements directly: compiled through a separate inlined function, the
+// 4096^2 FP64 column kernels needed more registers and spilled (P2 183 ->
+// 233 us per launch).
 template <class CT, class TS, int N, int MODE, class SC>
 __device__ __forceinline__ void col_body(const PassArgs& a0, int bx, int q, const cplx<CT>* __restrict__ tw_g,
-                                         char* smem) {
-  using Cfg = ColCfg<CT, TS, N, MODE, SC>;
-  using Sh = ColShared<Cfg, CT>;
-  const PassArgs a = instance_view(a0, q);
-  constexpr int E = Cfg::E, TPL = Cfg::TPL, C = Cfg::C, LP = Cfg::LP, NT = Cfg::NT;
-  constexpr bool WAVE = Cfg::WAVE;
-  constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
-  constexpr bool ST = Cfg::ST;
-  cplx<CT>* lds = reinterpret_cast<cplx<CT>*>(smem);
-  cplx<CT>* tw_s = reinterpret_cast<cplx<CT>*>(smem + Sh::OFF_TW);   // the stage tables, or the half table
-  double* scratch = reinterpret_cast<double*>(smem + Sh::OFF_SCR);
-  const cplx<CT>* tw = (ST || tw_in_lds<N, CT>()) ? tw_s : tw_g;
-  if constexpr (MODE == COL_P4) {
-    // the partials of P1..P3 of this step are complete: history row
-    if (bx == 0) {
-      if (a.fin.hist != nullptr) finalize_block<NT>(a.fin, scratch);
-      return;
-    }
-  }
-  const int lb = MODE == COL_P4 ? bx - 1 : bx;
-  const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
-  const int kx = lb * C + l;
-  cplx<CT>* line = lds + l * LP;
-  const size_t base = (size_t)kx * N;
-  const DevParams& p = a.p;
-  const double inv_nsp = p.inv_nsp;
-  const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);  // (pi/spacing)^2
-
-  cplx<CT> v[2][E];
-  cplx<TS> sp[E];   // P4: the state line, loaded with the inputs
-  if constexpr (MODE == COL_P2 || MODE == COL_MU) {
-    const cplx<TS>* S = static_cast<const cplx<TS>*>(a.in0);
-    const cplx<CT>* F = static_cast<const cplx<CT>*>(a.in1);
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      v[0][j] = cvt<CT>(load_line(&S[base + t + TPL * j], a.nt));
-      v[1][j] = load_line(&F[base + t + TPL * j], a.nt);
-    }
-  } else {
-    const cplx<CT>* X0 = static_cast<const cplx<CT>*>(a.in0);
-    const cplx<CT>* X1 = static_cast<const cplx<CT>*>(a.in1);
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      v[0][j] = load_line(&X0[base + t + TPL * j], a.nt);
-      v[1][j] = load_line(&X1[base + t + TPL * j], a.nt);
-    }
-    if constexpr (MODE == COL_P4 && Cfg::PF) {
-      const cplx<TS>* S = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
-#pragma unroll
-      for (int j = 0; j < E; ++j) sp[j] = load_line(&S[base + t + TPL * j], a.nt);
-    }
-  }
-  // twiddles after the line loads: a single wait covers both
-  if constexpr (ST) load_stage_tables<N, N, E, NT>(tw_s, tw_g);
-  else if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
-  __syncthreads();
-  fft_regs<CT, -1, N, 2, N, WAVE, E, XM, AS, BS, ST>(v, t, line, tw);
-
-  // packed line 0 (k_x = 0 in re, k_x = N/2 in im) needs the value at -ky of
-  // one array: stage it through the line buffer
-  cplx<CT> partner[E];
-  if (kx == 0) {
-    constexpr int PA = (MODE == COL_P2 || MODE == COL_MU) ? 0 : 1;
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int j = 0; j < E; ++j) line[pad<CT, N>(t + TPL * j)] = v[PA][j];
-    lds_sync<WAVE>();
-#pragma unroll
-    for (int j = 0; j < E; ++j) partner[j] = line[pad<CT, N>(neg_index<N>(t + TPL * j))];
-    lds_sync<WAVE>();
-  }
-
-  if constexpr (MODE == COL_P2 || MODE == COL_MU) {
-    double esum = 0.0;
-    const double tk = 2.0 * p.kappa;
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int ky = t + TPL * j;
-      const cplx<CT> ch = v[0][j], fh = v[1][j];
-      const double kyv = kval<N>(ky, inv_nsp);
-      const double ky2 = kyv * kyv;
-      cplx<CT> mu;
-      if (kx == 0) {
-        // A part: k_x = 0; B part: k_x = N/2 (Nyquist, kept in k^2)
-        const cplx<CT> chn = partner[j];
-        const CT sa = static_cast<CT>(tk * (ky2 + 0.5 * kN2));   // (a+b)/2
-        const CT sd = static_cast<CT>(-0.5 * tk * kN2);           // (a-b)/2
-        mu = fh + sa * ch + sd * conj(chn);
-        const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
-        const double re = (double)ch.x * chn.x - (double)ch.y * chn.y;
-        esum += (ky2 + 0.5 * kN2) * m2 - 0.5 * kN2 * re;   // K_A^2|A|^2 + K_B^2|B|^2
-      } else {
-        const double kxv = kval<N>(kx, inv_nsp);
-        const double k2 = kxv * kxv + ky2;   // core_shell.jl:23
-        mu = fh + static_cast<CT>(tk * k2) * ch;
-        const double m2 = (double)ch.x * ch.x + (double)ch.y * ch.y;
-        esum += 2.0 * k2 * m2;   // Hermitian weight of interior k_x
-      }
-      if constexpr (MODE == COL_MU) {
-        v[0][j] = static_cast<CT>(1.0 / N) * mu;
-      } else {
-        const CT sx = static_cast<CT>(dkval<N>(kx, inv_nsp) / N);   // 0 on line 0
-        const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp) / N);
-        v[0][j] = mul_i(sx * mu);
-        v[1][j] = mul_i(sy * mu);
-      }
-    }
-    if constexpr (MODE == COL_P2) {
-      fft_regs<CT, +1, N, 2, N, WAVE, E, XM, AS, BS, ST>(v, t, line, tw);
-      cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
-      cplx<CT>* O1 = static_cast<cplx<CT>*>(a.out1);
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        store_out(&O0[base + t + TPL * j], v[0][j], a.nt);
-        store_out(&O1[base + t + TPL * j], v[1][j], a.nt);
-      }
-      double r[1] = {esum};
-      const int ops[1] = {RED_SUM};
-      block_reduce<NT, 1>(r, ops, scratch);
-      if (threadIdx.x == 0) a.part[bx] = r[0];
-    } else {
-      cplx<CT> w[1][E];
-#pragma unroll
-      for (int j = 0; j < E; ++j) w[0][j] = v[0][j];
-      fft_regs<CT, +1, N, 1, N, WAVE, E, XM, AS, BS, ST>(w, t, line, tw);
-      cplx<CT>* O0 = static_cast<cplx<CT>*>(a.out0);
-#pragma unroll
-      for (int j = 0; j < E; ++j) O0[base + t + TPL * j] = w[0][j];
-    }
-    return;
-  }
-
-  if constexpr (MODE == COL_P4 || MODE == COL_RATE) {
-    const double dt = a.dt;
-    const double qk2dt = 2.0 * p.Q * p.kappa;   // core_shell.jl:74  2p.Q*p.kappa*k2^2*dt
-    cplx<CT> w[1][E];
-    cplx<double>* R = static_cast<cplx<double>*>(a.out0);
-#pragma unroll
-    for (int j = 0; j < E; ++j) {
-      const int ky = t + TPL * j;
-      const cplx<CT> jx = v[0][j], jy = v[1][j];
-      const double kyv = kval<N>(ky, inv_nsp);
-      const double ky2 = kyv * kyv;
-      const CT sy = static_cast<CT>(dkval<N>(ky, inv_nsp));
-      if (kx == 0) {
-        const int kn = neg_index<N>(ky);
-        const CT syn = static_cast<CT>(dkval<N>(kn, inv_nsp));
-        // rate on the packed line: i*kdy*Jy^ (k_dx = 0 for k_x = 0 and N/2)
-        const cplx<CT> r = mul_i(sy * jy);
-        const cplx<CT> rn = mul_i(syn * partner[j]);
-        if constexpr (MODE == COL_RATE) {
-          // unpack: A = (r + conj rn)/2 (k_x = 0), B = (r - conj rn)/(2i) (k_x = N/2)
-          R[ky] = cvt<double>(static_cast<CT>(0.5) * (r + conj(rn)));
-          R[(size_t)(N / 2) * N + ky] = cvt<double>(static_cast<CT>(0.5) * mul_mi(r - conj(rn)));
-        } else {
-          const double k2a = ky2, k2b = kN2 + ky2;
-          const double ia = 1.0 / (1.0 + qk2dt * (k2a * k2a) * dt);
-          const double ib = 1.0 / (1.0 + qk2dt * (k2b * k2b) * dt);
-          const CT s = static_cast<CT>(dt / N);
-          const CT ha = static_cast<CT>(0.5 * (ia + ib)), hd = static_cast<CT>(0.5 * (ia - ib));
-          w[0][j] = s * (ha * r + hd * conj(rn));
-        }
-      } else {
-        const double kxv = kval<N>(kx, inv_nsp);
-        const CT sx = static_cast<CT>(kxv);   // interior k_x: no Nyquist
-        const cplx<CT> r = mul_i(sx * jx) + mul_i(sy * jy);
-        if constexpr (MODE == COL_RATE) {
-          R[base + ky] = cvt<double>(r);
-        } else {
-          const double k2 = kxv * kxv + ky2;
-          const double den = 1.0 + qk2dt * (k2 * k2) * dt;
-          w[0][j] = static_cast<CT>(dt / (den * N)) * r;
-        }
-      }
-    }
-    if constexpr (MODE == COL_P4) {
-      fft_regs<CT, +1, N, 1, N, WAVE, E, XM, AS, BS, ST>(w, t, line, tw);
-      // The increment's k = 0 mode is exactly zero (kd = 0 there), so the
-      // real part of packed line 0 (k_x = 0) sums to zero over j and the mean
-      // composition sum_j Re S[0][j] / N^2 is conserved (core_shell.jl:74-75).
-      // An FP32 inverse transform leaves that sum at O(eps_32 |inc|); the
-      // mixed form removes it in FP64 so the mean changes only by FP64
-      // rounding.  The FP64 form needs no correction.
-      double dmean = 0.0;
-      if constexpr (!std::is_same<CT, double>::value) {
-        if (lb == 0) {   // uniform: the workgroup that holds line k_x = 0
-          double r[1] = {0.0};
-          if (kx == 0) {
-#pragma unroll
-            for (int j = 0; j < E; ++j) r[0] += static_cast<double>(w[0][j].x);
-          }
-          const int ops[1] = {RED_SUM};
-          block_reduce<NT, 1>(r, ops, scratch);
-          dmean = (kx == 0) ? r[0] / N : 0.0;
-        }
-      }
-      // S += IFFT_j(inc)
-      cplx<TS>* S = static_cast<cplx<TS>*>(a.out0);
-      if constexpr (!Cfg::PF) {
-        const cplx<TS>* Sin = static_cast<const cplx<TS>*>(a.sin ? a.sin : a.out0);
-#pragma unroll
-        for (int j = 0; j < E; ++j) sp[j] = load_line(&Sin[base + t + TPL * j], a.nt);
-      }
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        cplx<TS> s = sp[j];
-        s.x += static_cast<TS>(static_cast<double>(w[0][j].x) - dmean);
-        s.y += static_cast<TS>(w[0][j].y);
-        store_out(&S[base + t + TPL * j], s, a.nt);
-      }
-    }
-  }
+                                         cplx<CT>* lds, cplx<CT>* tw_s, double* scratch) {
+#include "csg_col_body.inc"
 }
 
 template <class CT, class TS, int N, int MODE, class SC = Sched<>>
 __global__ void __launch_bounds__((ColCfg<CT, TS, N, MODE, SC>::NT))
 col_pass(PassArgs a0, const cplx<CT>* __restrict__ tw_g) {
-  using Cfg = ColCfg<CT, TS, N, MODE, SC>;
-  __shared__ __attribute__((aligned(16))) char smem[ColShared<Cfg, CT>::BYTES];
-  col_body<CT, TS, N, MODE, SC>(a0, blockIdx.x, blockIdx.y, tw_g, smem);
+  using Sh = ColShared<ColCfg<CT, TS, N, MODE, SC>, CT>;
+  __shared__ cplx<CT> lds[Sh::NLDS];
+  __shared__ cplx<CT> tw_s[Sh::NTW > 0 ? Sh::NTW : 1];
+  __shared__ double scratch[Sh::NSCR];
+  const int bx = static_cast<int>(blockIdx.x), q = static_cast<int>(blockIdx.y);
+#include "csg_col_body.inc"
 }
 
 // ---------------------------------------------------------------------------

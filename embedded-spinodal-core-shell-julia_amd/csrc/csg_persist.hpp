@@ -178,15 +178,34 @@ __device__ __forceinline__ void persist_item(ConstClass* k, int pass, int bx, in
   constexpr bool F64 = std::is_same<T, double>::value;
   const cplx<T>* tw = static_cast<const cplx<T>*>(k->tw);
   const PassArgs* args = (const PassArgs*)(k->args);
+  // the item's LDS regions, carved from the kernel's one buffer
+  auto row = [&](auto cfg, auto fn) {
+    using Sh = RowShared<decltype(cfg), T>;
+    fn(reinterpret_cast<cplx<T>*>(smem), reinterpret_cast<cplx<T>*>(smem + Sh::OFF_TW),
+       reinterpret_cast<cplx<T>*>(smem + Sh::OFF_ST), reinterpret_cast<double*>(smem + Sh::OFF_SCR));
+  };
+  auto col = [&](auto cfg, auto fn) {
+    using Sh = ColShared<decltype(cfg), T>;
+    fn(reinterpret_cast<cplx<T>*>(smem), reinterpret_cast<cplx<T>*>(smem + Sh::OFF_TW),
+       reinterpret_cast<double*>(smem + Sh::OFF_SCR));
+  };
   if (pass == 0) {
-    row_body<T, double, T, N, ROW_P1, typename PSched<N, F64, 1>::type>(args[0], bx, q, tw, smem);
+    row(PRowCfg<T, N, 1>{}, [&](cplx<T>* l, cplx<T>* t, cplx<T>* st, double* sc) {
+      row_body<T, double, T, N, ROW_P1, typename PSched<N, F64, 1>::type>(args[0], bx, q, tw, l, t, st, sc);
+    });
   } else if (pass == 1) {
-    col_body<T, double, N, COL_P2, typename PSched<N, F64, 2>::type>(args[1], bx, q, tw, smem);
+    col(PColCfg<T, N, 2>{}, [&](cplx<T>* l, cplx<T>* t, double* sc) {
+      col_body<T, double, N, COL_P2, typename PSched<N, F64, 2>::type>(args[1], bx, q, tw, l, t, sc);
+    });
   } else if (pass == 2) {
-    row_body<T, T, T, N, ROW_P3, typename PSched<N, F64, 3>::type>(args[2], bx, q, tw, smem);
+    row(PRowCfg<T, N, 3>{}, [&](cplx<T>* l, cplx<T>* t, cplx<T>* st, double* sc) {
+      row_body<T, T, T, N, ROW_P3, typename PSched<N, F64, 3>::type>(args[2], bx, q, tw, l, t, st, sc);
+    });
   } else if (advance || bx == 0) {
     // the last iteration of a run writes its history row without a step
-    col_body<T, double, N, COL_P4, typename PSched<N, F64, 4>::type>(args[3], bx, q, tw, smem);
+    col(PColCfg<T, N, 4>{}, [&](cplx<T>* l, cplx<T>* t, double* sc) {
+      col_body<T, double, N, COL_P4, typename PSched<N, F64, 4>::type>(args[3], bx, q, tw, l, t, sc);
+    });
   }
 }
 
