@@ -71,7 +71,26 @@ def _args():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 times), the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--counters", type=int, default=0, metavar="STEPS",
+                    help="only run STEPS eager loop iterations (no graph capture) of each form at "
+                         "the --n field and at 4096^2 with the default schedules: the workload of "
+                         "the rocprofv3 --pmc passes, whose counter collection does not follow "
+                         "kernels launched from captured graphs")
     return ap.parse_args()
+
+
+def counter_pass(pkg, a) -> None:
+    """Eager loop iterations (csg_step: one launch per pass, no graph) of both
+    forms on the --n workload and on the 4096^2 large field."""
+    for n, radius, p in ((a.n, a.radius, pkg.Parameters()), (4096, 1300.0, janus_params(pkg))):
+        c0, phi = pkg.initial_fields(n, radius=radius, width=a.width, seed=5749)
+        for form in ("fp64", "mixed"):
+            s = pkg.Session(n, 1.0, p, form)
+            s.set_fields(c0, phi)
+            s.step(a.dt, a.counters)
+            print(json.dumps({"counters_workload": f"{n}x{n}", "form": form, "steps": a.counters,
+                              "schedule": list(s.schedule)}), flush=True)
+            s.close()
 
 
 def calibrate_copy(mib: int) -> None:
@@ -441,6 +460,10 @@ def main():
     a = _args()
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.counters:
+        import __graft_entry__
+        counter_pass(__graft_entry__.load_package(), a)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the library has no "

@@ -49,8 +49,7 @@ struct csg_ctx {
   long long* ctr = nullptr;
   double* hist = nullptr;
   size_t hist_cap = 0;    // rows
-  long long* hpin = nullptr;  // pinned host staging of ctr + history rows
-  size_t hpin_bytes = 0;
+  std::vector<long long> hstage;   // host staging of ctr (2 words) + history rows
   double* scratch_row = nullptr;
   double* rate = nullptr; // (N/2+1) x N complex doubles
   bool fields_set = false;
@@ -257,16 +256,10 @@ int build_graph(csg_ctx* c, double dt, int64_t every) {
   return CSG_OK;
 }
 
-// Pinned host staging for ctr (2 words) and `rows` history rows.
-int ensure_hpin(csg_ctx* c, size_t rows) {
-  const size_t need = 2 * sizeof(long long) + rows * 6 * sizeof(double);
-  if (need <= c->hpin_bytes) return CSG_OK;
-  if (c->hpin) HIPCHK(hipHostFree(c->hpin));
-  c->hpin = nullptr;
-  c->hpin_bytes = 0;
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hpin), need, hipHostMallocDefault));
-  c->hpin_bytes = need;
-  return CSG_OK;
+// Host staging for ctr (2 words) and `rows` history rows, kept between runs.
+void ensure_hstage(csg_ctx* c, size_t rows) {
+  const size_t need = 2 + rows * 6;   // 8-byte words
+  if (c->hstage.size() < need) c->hstage.resize(need);
 }
 
 int check_ctx(const csg_ctx* c) {
@@ -366,7 +359,6 @@ int csg_destroy(csg_ctx* c) {
     if (b) (void)hipFree(b);
   if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
   if (c->stream) (void)hipStreamDestroy(c->stream);
-  if (c->hpin) (void)hipHostFree(c->hpin);
   delete c;
   return CSG_OK;
 }
@@ -415,7 +407,7 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
   // the device keeps rows s = 0, hist_every, 2 hist_every, ... only
   const int64_t rows = steps / hist_every + 1;
   if (int rc = ensure_hist(c, static_cast<size_t>(rows))) return rc;
-  if (int rc = ensure_hpin(c, history ? static_cast<size_t>(rows) : 0)) return rc;
+  ensure_hstage(c, history ? static_cast<size_t>(rows) : 0);
   if (int rc = reset_ctr(c)) return rc;
   int64_t done = 0;
   if (steps >= 2 * STEPS_PER_GRAPH) {
@@ -428,8 +420,8 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
   for (; done < steps; ++done) enqueue_iteration(c, dt, true, hist_every);
   enqueue_iteration(c, dt, false, hist_every);
   HIPCHK(hipGetLastError());
-  long long* ctr = c->hpin;
-  double* h = reinterpret_cast<double*>(c->hpin + 2);
+  long long* ctr = c->hstage.data();
+  double* h = reinterpret_cast<double*>(c->hstage.data() + 2);
   HIPCHK(hipMemcpyAsync(ctr, c->ctr, 2 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   if (history)
     HIPCHK(hipMemcpyAsync(h, c->hist, static_cast<size_t>(rows) * 6 * sizeof(double), hipMemcpyDeviceToHost,
