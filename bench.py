@@ -488,7 +488,7 @@ def main():
         o = dict(value=ups, ms_per_step=1e3 * r["seconds"] / (a.steps + 1), steps_per_s=(a.steps + 1) / r["seconds"],
                  pass_ms=dict(zip(PASS_NAMES + ("finalize",), [float(x) for x in ms])),
                  pass_gbs=dict(zip(PASS_NAMES, per)),
-                 step_gbs=step_bytes / (sum(ms) * 1e-3) / 1e9,
+                 step_gbs=step_bytes * (a.steps + 1) / r["seconds"] / 1e9,
                  dominant=PASS_NAMES[dom], dominant_gbs=per[dom], dominant_ms=float(ms[dom]),
                  dominant_bytes=PASS_BYTES[f][dom] * sites, dominant_schedule=r["resolved"][dom],
                  energy_final=float(r["hist"][-1, 2]),

@@ -50,6 +50,9 @@ struct csg_ctx {
   double* hist = nullptr;
   size_t hist_cap = 0;    // rows
   std::vector<long long> hstage;   // host staging of ctr (2 words) + history rows
+  // non-finite checks during long runs: pinned slots for ctr[1] and events
+  long long* flag_pin = nullptr;
+  hipEvent_t flag_ev[2] = {nullptr, nullptr};
   double* scratch_row = nullptr;
   double* rate = nullptr; // (N/2+1) x N complex doubles
   bool fields_set = false;
@@ -257,6 +260,18 @@ int build_graph(csg_ctx* c, double dt, int64_t every) {
 }
 
 // Host staging for ctr (2 words) and `rows` history rows, kept between runs.
+// Pinned host slots and events for the non-finite checks of csg_run (made
+// on first use; a context that never runs a long loop allocates none).
+int ensure_flag_slots(csg_ctx* c) {
+  if (c->flag_pin) return CSG_OK;
+  for (hipEvent_t& ev : c->flag_ev)
+    if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  void* p = nullptr;
+  HIPCHK(hipHostMalloc(&p, 2 * sizeof(long long), hipHostMallocDefault));
+  c->flag_pin = static_cast<long long*>(p);
+  return CSG_OK;
+}
+
 void ensure_hstage(csg_ctx* c, size_t rows) {
   const size_t need = 2 + rows * 6;   // 8-byte words
   if (c->hstage.size() < need) c->hstage.resize(need);
@@ -358,6 +373,9 @@ int csg_destroy(csg_ctx* c) {
   for (void* b : c->seg)
     if (b) (void)hipFree(b);
   if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
+  if (c->flag_pin) (void)hipHostFree(c->flag_pin);
+  for (hipEvent_t ev : c->flag_ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CSG_OK;
@@ -410,15 +428,42 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
   ensure_hstage(c, history ? static_cast<size_t>(rows) : 0);
   if (int rc = reset_ctr(c)) return rc;
   int64_t done = 0;
+  bool stopped = false;
   if (steps >= 2 * STEPS_PER_GRAPH) {
     if (int rc = build_graph(c, dt, hist_every)) return rc;
+    // Long runs check the device's first-non-finite step between blocks of
+    // CHECK_GRAPHS replays, one block behind the one being enqueued, so the
+    // queue never drains and a run that went non-finite stops within two
+    // blocks instead of computing every remaining step.
+    const bool check = steps >= 2 * CHECK_GRAPHS * STEPS_PER_GRAPH;
+    if (check) {
+      if (int rc = ensure_flag_slots(c)) return rc;
+      c->flag_pin[0] = c->flag_pin[1] = -1;
+    }
+    int64_t block = 0;
     while (steps - done >= STEPS_PER_GRAPH) {
       HIPCHK(hipGraphLaunch(c->gexec, c->stream));
       done += STEPS_PER_GRAPH;
+      if (check && (done / STEPS_PER_GRAPH) % CHECK_GRAPHS == 0) {
+        const int slot = static_cast<int>(block & 1);
+        HIPCHK(hipMemcpyAsync(c->flag_pin + slot, c->ctr + 1, sizeof(long long), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipEventRecord(c->flag_ev[slot], c->stream));
+        if (block > 0) {
+          HIPCHK(hipEventSynchronize(c->flag_ev[slot ^ 1]));
+          if (c->flag_pin[slot ^ 1] >= 0) {
+            stopped = true;
+            break;
+          }
+        }
+        ++block;
+      }
     }
   }
-  for (; done < steps; ++done) enqueue_iteration(c, dt, true, hist_every);
-  enqueue_iteration(c, dt, false, hist_every);
+  if (!stopped) {
+    for (; done < steps; ++done) enqueue_iteration(c, dt, true, hist_every);
+    enqueue_iteration(c, dt, false, hist_every);
+  }
   HIPCHK(hipGetLastError());
   long long* ctr = c->hstage.data();
   double* h = reinterpret_cast<double*>(c->hstage.data() + 2);

@@ -97,5 +97,6 @@ inline int nontemporal_for(size_t bytes, bool ensemble) {
 
 constexpr int FIN_NT = 256;           // threads of the standalone history reduction
 constexpr int STEPS_PER_GRAPH = 32;   // loop iterations per captured graph
+constexpr int CHECK_GRAPHS = 64;      // graph replays between non-finite checks of a long run
 
 }  // namespace csg

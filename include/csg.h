@@ -69,8 +69,11 @@ int csg_set_fields(csg_ctx* ctx, const double* c0, const double* phi);
  * history: caller-allocated, (rows x 6) column-major float64 with
  * rows = steps / hist_every + 1 (rows s = 0, hist_every, 2*hist_every, ...);
  * may be NULL.  Columns: [t, mean c, F, min c, max c, dissipation].
- * Returns CSG_ERR_NONFINITE (state left at c_steps) if any c_s is
- * non-finite; csg_nonfinite_step then gives the first such s. */
+ * Returns CSG_ERR_NONFINITE if any c_s is non-finite; csg_nonfinite_step
+ * then gives the first such s, history rows up to it are valid, and the
+ * resident field is unspecified (a run of >= 4096 steps checks the device
+ * flag every 2048 steps and stops within 4096 steps of s; a shorter run
+ * completes its steps first). */
 int csg_run(csg_ctx* ctx, double dt, int64_t steps, int64_t hist_every, double* history);
 
 /* nsteps applications of step (core_shell.jl:72-76) to the resident field. */

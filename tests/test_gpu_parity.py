@@ -202,6 +202,22 @@ def test_mixed_mass_drift_fp64_level(pkg, n, steps):
     assert np.max(np.diff(hist[:, 2])) < 1e-8
 
 
+def test_mixed_horizon_1024(pkg):
+    """Mixed against FP64 on the 1024^2 plain-method workload (R=320, w=2)
+    after 1000 steps.  The model amplifies any perturbation by ~1e3 over this
+    horizon (an FP64 c0 perturbation of 1e-8 grows to ~9e-6), so the bound is
+    2e-5 on max|dc| (measured 9.5e-6 to 1.13e-5 across kernel layouts) and
+    1e-6 on the relative energy."""
+    say("mixed vs fp64 1024^2, 1000 steps")
+    c0, phi = _fields(pkg, 1024, radius=320.0, width=2.0)
+    f64, h64 = pkg.run_simulation(c0, phi, dt=0.01, steps=1000, history_every=100)
+    fmx, hmx = pkg.run_simulation(c0, phi, dt=0.01, steps=1000, history_every=100, precision="mixed")
+    dc = np.max(np.abs(fmx - f64))
+    assert dc <= 2e-5, dc
+    assert abs(hmx[-1, 2] / h64[-1, 2] - 1) <= 1e-6
+    assert np.max(np.abs(hmx[:, 1] - h64[0, 1])) <= 1e-13
+
+
 def test_checks_jl_identities_on_gpu(pkg):
     say("checks.jl identities against the GPU backend")
     c, phi = _fields(pkg, 64, radius=16.0)
@@ -261,6 +277,25 @@ def test_nonfinite_blowup_step_matches_oracle(pkg, tmp_path):
         pkg.run_simulation(c0, phi, dt=0.01, steps=200, directory=str(tmp_path / "out"))
     assert str(eg.value) == str(er.value)
     assert not (tmp_path / "out").exists()   # no files on error (core_shell.jl:93-101)
+
+
+def test_nonfinite_long_run_stops_early(pkg):
+    """A long captured run that turns non-finite stops within two check blocks
+    (2 x 64 graph replays of 32 steps) of the first non-finite step, as the
+    reference's loop stops at once (core_shell.jl:93): one million steps at
+    32^2 take ~10 s in full, the stopped run well under a second."""
+    import time
+    say("non-finite blow-up in a 1e6-step run")
+    c0, phi = _fields(pkg, 32)
+    c0[10, 10] = 3.0e3
+    with pytest.raises(pkg.NonFiniteCompositionError) as short:
+        pkg.run_simulation(c0, phi, dt=0.01, steps=200)
+    t0 = time.perf_counter()
+    with pytest.raises(pkg.NonFiniteCompositionError) as long_run:
+        pkg.run_simulation(c0, phi, dt=0.01, steps=1_000_000, history_every=1000)
+    secs = time.perf_counter() - t0
+    assert long_run.value.step == short.value.step
+    assert secs < 3.0, secs
 
 
 def test_results_directory(pkg, tmp_path):
