@@ -378,6 +378,19 @@ __device__ __forceinline__ void load_stage_tables(cplx<T>* st, const cplx<T>* __
   }
 }
 
+// Workgroup barrier for LDS traffic only: the wave's LDS accesses complete
+// (s_waitcnt lgkmcnt(0); the vmcnt and expcnt fields are left at their
+// no-wait maxima), then s_barrier.  Wavefront-scope fences keep the compiler
+// from moving memory accesses across it.  Unlike __syncthreads() it does not
+// wait for the wave's outstanding global loads, so loads issued before an
+// exchange (a prefetched phi row or state line) stay in flight through it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Order LDS traffic between stages: lines that fit in one wavefront only need
 // the wave's own accesses ordered; wider lines need the workgroup barrier.
 template <bool WAVE>
@@ -387,7 +400,7 @@ __device__ __forceinline__ void lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   } else {
-    __syncthreads();
+    lds_barrier();
   }
 }
 

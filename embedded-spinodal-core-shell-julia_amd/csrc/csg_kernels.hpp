@@ -311,6 +311,95 @@ struct RowShared {
   static constexpr int BYTES = OFF_SCR + NSCR * (int)sizeof(double);
 };
 
+// P1 pointwise work at a thread's E site pairs (c in v[j].x / .y, phi in
+// ph[j][0 / 1]): v becomes f'(c) (bulk_derivative, core_shell.jl:53-54);
+// the bulk energy sum, min c, max c and the non-finite count go to red[0],
+// red[2], red[3], red[4].  Non-finite values are counted and kept out of
+// min/max (branchless).
+template <class CT, int E>
+__device__ __forceinline__ void p1_sites(cplx<CT> (&v)[E], const CT (&ph)[E][2], const DevParams& p, double (&red)[5]) {
+  const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
+  double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const CT cin[2] = {v[j].x, v[j].y};
+    const CT qq[2] = {ph[j][0], ph[j][1]};
+    CT out[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const CT c = cin[e], q = qq[e];
+      const CT h = q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q);   // core_shell.jl:40
+      const CT g = q * q * (CT(1) - q) * (CT(1) - q);                     // :41
+      // bulk_derivative, core_shell.jl:53-54
+      out[e] = CT(2) * A * (CT(1) - h) * (c - ca) +
+               CT(2) * Bc * h * (c - cb) * (c - cc) * (CT(2) * c - cb - cc) - chi * P * g;
+      // bulk_energy, core_shell.jl:47-48
+      const CT fb = A * (CT(1) - h) * (c - ca) * (c - ca) +
+                    Bc * h * (c - cb) * (c - cb) * (c - cc) * (c - cc) + P * (CT(1) - chi * c) * g;
+      fsum += static_cast<double>(fb);
+      const bool ok = finite(c);
+      const double cd = static_cast<double>(c);
+      cmin = fmin(cmin, ok ? cd : INFINITY);
+      cmax = fmax(cmax, ok ? cd : -INFINITY);
+      nbad += ok ? 0.0 : 1.0;
+    }
+    v[j] = mk(out[0], out[1]);
+  }
+  red[0] += fsum;
+  red[2] = fmin(red[2], cmin);
+  red[3] = fmax(red[3], cmax);
+  red[4] += nbad;
+}
+
+// P3 pointwise work: J = M0 h(phi) g for each of the NARR gradient
+// components in v (core_shell.jl:42, 66-67); the dissipation sum
+// sum(M (gx^2 + gy^2)) (:68) goes to red[0].
+template <class CT, int NARR, int E>
+__device__ __forceinline__ void p3_sites(cplx<CT> (&v)[NARR][E], const CT (&ph)[E][2], const DevParams& p,
+                                         double (&red)[5]) {
+  const CT mob = p.mobility;
+  double dsum = 0.0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const CT q0 = ph[j][0], q1 = ph[j][1];
+    const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
+    const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
+#pragma unroll
+    for (int q = 0; q < NARR; ++q) {
+      const cplx<CT> g = v[q][j];
+      dsum += static_cast<double>(M0 * (g.x * g.x)) + static_cast<double>(M1 * (g.y * g.y));
+      v[q][j] = mk(M0 * g.x, M1 * g.y);
+    }
+  }
+  red[0] += dsum;
+}
+
+// R2C post-processing of one row after the forward FFT along i: X_k = E_k +
+// W_N^k O_k from (Z_k, Z_{L-k}), slot 0 packing (X_0, X_L).  `tr` is the
+// row's LDS line buffer (padded layout); both orderings points are included.
+template <class CT, int L, int E, bool WAVE>
+__device__ __forceinline__ void r2c_post(cplx<CT> (&v)[E], cplx<CT>* tr, int t, const cplx<CT>* tw) {
+  constexpr int TPL = L / E;
+#pragma unroll
+  for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[j];
+  lds_sync<WAVE>();
+  const CT half = static_cast<CT>(0.5);
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int k = t + TPL * j;
+    const cplx<CT> zk = v[j];
+    if (k == 0) {
+      v[j] = mk(zk.x + zk.y, zk.x - zk.y);
+    } else {
+      const cplx<CT> zm = tr[pad<CT, L>(L - k)];
+      const cplx<CT> ek = half * (zk + conj(zm));
+      const cplx<CT> ok = half * mul_mi(zk - conj(zm));
+      v[j] = ek + cmul(tw[k], ok);
+    }
+  }
+  lds_sync<WAVE>();
+}
+
 // Body of a row pass for tile-workgroup `bx` of instance q, with its LDS
 // regions (tile and exchange buffers, twiddle half table, stage tables,
 // reduction scratch; RowShared sizes): the batched kernel below runs it for
@@ -422,86 +511,18 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   // then R2C along i (forward FFT, X_k = E_k + W_N^k O_k from (Z_k, Z_{L-k}))
   // and the transposed store of the tile to line layout
   auto pointwise_r2c_store = [&](int arr0, int j0, const CT(&ph)[E][2]) {
-    if constexpr (MODE == ROW_P1) {
-      const DevParams& p = a.p;
-      const CT A = p.A, Bc = p.B, chi = p.chi, P = p.P, ca = p.ca, cb = p.cb, cc = p.cc;
-      double fsum = 0.0, cmin = INFINITY, cmax = -INFINITY, nbad = 0.0;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const CT cin[2] = {v[0][j].x, v[0][j].y};
-        const CT qq[2] = {ph[j][0], ph[j][1]};
-        CT out[2];
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const CT c = cin[e], q = qq[e];
-          const CT h = q * q * q * (CT(10) - CT(15) * q + CT(6) * q * q);   // core_shell.jl:40
-          const CT g = q * q * (CT(1) - q) * (CT(1) - q);                     // :41
-          // bulk_derivative, core_shell.jl:53-54
-          out[e] = CT(2) * A * (CT(1) - h) * (c - ca) +
-                   CT(2) * Bc * h * (c - cb) * (c - cc) * (CT(2) * c - cb - cc) - chi * P * g;
-          // bulk_energy, core_shell.jl:47-48
-          const CT fb = A * (CT(1) - h) * (c - ca) * (c - ca) +
-                        Bc * h * (c - cb) * (c - cb) * (c - cc) * (c - cc) + P * (CT(1) - chi * c) * g;
-          fsum += static_cast<double>(fb);
-          // branchless: non-finite values are counted and kept out of min/max
-          const bool ok = finite(c);
-          const double cd = static_cast<double>(c);
-          cmin = fmin(cmin, ok ? cd : INFINITY);
-          cmax = fmax(cmax, ok ? cd : -INFINITY);
-          nbad += ok ? 0.0 : 1.0;
-        }
-        v[0][j] = mk(out[0], out[1]);
-      }
-      red[0] += fsum;
-      red[2] = fmin(red[2], cmin);
-      red[3] = fmax(red[3], cmax);
-      red[4] += nbad;
-    }
-    if constexpr (MODE == ROW_P3) {
-      const CT mob = a.p.mobility;
-      double dsum = 0.0;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const CT q0 = ph[j][0], q1 = ph[j][1];
-        const CT M0 = mob * (q0 * q0 * q0 * (CT(10) - CT(15) * q0 + CT(6) * q0 * q0));   // :42
-        const CT M1 = mob * (q1 * q1 * q1 * (CT(10) - CT(15) * q1 + CT(6) * q1 * q1));
-#pragma unroll
-        for (int q = 0; q < NARR; ++q) {
-          const cplx<CT> g = v[q][j];
-          // core_shell.jl:68 sum(M .* (gx.^2 .+ gy.^2))
-          dsum += static_cast<double>(M0 * (g.x * g.x)) + static_cast<double>(M1 * (g.y * g.y));
-          v[q][j] = mk(M0 * g.x, M1 * g.y);
-        }
-      }
-      red[0] += dsum;
-    }
+    if constexpr (MODE == ROW_P1) p1_sites<CT, E>(v[0], ph, a.p, red);
+    if constexpr (MODE == ROW_P3) p3_sites<CT, NARR, E>(v, ph, a.p, red);
     fft_regs<CT, -1, L, NARR, N, WAVE, E, XM, AS, BS, ST>(v, t, row, ftw);
 #pragma unroll
-    for (int q = 0; q < NARR; ++q) {
-      cplx<CT>* tr = lds + (q * B + b) * LP;
+    for (int q = 0; q < NARR; ++q) r2c_post<CT, L, E, WAVE>(v[q], lds + (q * B + b) * LP, t, tw);
+    if constexpr (B == 1 && NARR == 1) {
+      // one row: the thread's own elements k = t + TPL j are its stores
+      cplx<TOUT>* X = static_cast<cplx<TOUT>*>(arr0 == 0 ? a.out0 : a.out1);
 #pragma unroll
-      for (int j = 0; j < E; ++j) tr[pad<CT, L>(t + TPL * j)] = v[q][j];
+      for (int j = 0; j < E; ++j) store_out(&X[(size_t)(t + TPL * j) * N + j0], cvt<TOUT>(v[0][j]), a.nt);
+      return;
     }
-    lds_sync<WAVE>();
-    const CT half = static_cast<CT>(0.5);
-#pragma unroll
-    for (int q = 0; q < NARR; ++q) {
-      const cplx<CT>* tr = lds + (q * B + b) * LP;
-#pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const int k = t + TPL * j;
-        const cplx<CT> zk = v[q][j];
-        if (k == 0) {
-          v[q][j] = mk(zk.x + zk.y, zk.x - zk.y);
-        } else {
-          const cplx<CT> zm = tr[pad<CT, L>(L - k)];
-          const cplx<CT> ek = half * (zk + conj(zm));
-          const cplx<CT> ok = half * mul_mi(zk - conj(zm));
-          v[q][j] = ek + cmul(tw[k], ok);
-        }
-      }
-    }
-    lds_sync<WAVE>();
 #pragma unroll
     for (int q = 0; q < NARR; ++q) {
       cplx<CT>* tr = lds + (q * B + b) * LP;
@@ -552,7 +573,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
         }
         tile_to_lds(g0, tl);
       }
-      __syncthreads();
+      lds_barrier();   // the prefetched phi row stays in flight
       c2r();
     }
 
