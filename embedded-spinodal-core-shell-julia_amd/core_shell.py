@@ -135,7 +135,8 @@ class Session:
     def schedule_info(self) -> list:
         """Per pass, the compiled candidates as dicts (E = elements per thread,
         W = rows or lines per workgroup, G = tile-load group, PF = prefetch,
-        X = FFT exchange layout, T = 1 for per-stage twiddle tables, S = 1
+        X = FFT exchange layout, T = 1 for per-stage twiddle tables and 2
+        for the half table staged in LDS beyond 16 KiB, S = 1
         when P3 gives each gradient component its own workgroups, 0 when one
         workgroup transforms both; 0 for the other passes)."""
         keys = ("E", "W", "G", "PF", "X", "T", "S")

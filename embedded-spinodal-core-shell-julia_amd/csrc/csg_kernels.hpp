@@ -264,7 +264,9 @@ constexpr int row_rows() {
 //   X   FFT exchange layout (fft_regs XM: 0 one array at a time, 1 all
 //       arrays together, 2 all together with two buffer sets); -1 = 0,
 //   T   FFT twiddles: 1 per-stage tables copied to LDS (lanes read
-//       consecutive entries), 0 the half table; -1 = 0,
+//       consecutive entries), 0 the half table (staged in LDS up to 16
+//       KiB, else read through the caches), 2 the half table staged in LDS
+//       whatever its size; -1 = 0,
 //   S   P3 only: 1 gives each gradient component its own workgroups (phi is
 //       then read once per component), 0 transforms both components of W
 //       rows in one workgroup (phi read once).
@@ -288,6 +290,7 @@ struct RowCfg {
   static constexpr int LP = tile_stride<CT>(L, B);   // row stride of the tile
   static constexpr int AS = B * LP, BS = NARR * B * LP;   // exchange strides
   static constexpr bool ST = SC::T == 1;
+  static constexpr bool TWL = tw_in_lds<N, CT>() || SC::T == 2;   // half table in LDS
   static_assert(!ST || stab_fits<N, CT, L, E>(), "stage tables do not fit this schedule");
   static constexpr int LDS = (XM == 2 ? 2 : 1) * NARR * B * LP;
   static constexpr int NT = B * TPL;
@@ -303,7 +306,7 @@ struct RowCfg {
 // reduction scratch.
 template <class Cfg, class CT>
 struct RowShared {
-  static constexpr int NLDS = Cfg::LDS, NTW = tw_in_lds<Cfg::NN, CT>() ? Cfg::NN / 2 : 0,
+  static constexpr int NLDS = Cfg::LDS, NTW = Cfg::TWL ? Cfg::NN / 2 : 0,
                        NST = Cfg::ST ? stab_len(Cfg::L, Cfg::E) : 0, NSCR = 8 * ((Cfg::NT + 63) / 64);
   static constexpr int OFF_TW = NLDS * (int)sizeof(cplx<CT>);
   static constexpr int OFF_ST = OFF_TW + NTW * (int)sizeof(cplx<CT>);
@@ -416,7 +419,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   constexpr bool WAVE = Cfg::WAVE;
   constexpr int XM = Cfg::XM, AS = Cfg::AS, BS = Cfg::BS;
   constexpr bool ST = Cfg::ST;
-  const cplx<CT>* tw = tw_in_lds<N, CT>() ? tw_s : tw_g;   // real-FFT pre/post twiddles
+  const cplx<CT>* tw = Cfg::TWL ? tw_s : tw_g;   // real-FFT pre/post twiddles
   const cplx<CT>* ftw = ST ? st_s : tw;                     // FFT engine twiddles
   const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
   cplx<CT>* row = lds + b * LP;                 // FFT exchange buffer of this row
@@ -555,7 +558,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
         const int k = t + TPL * j;
         v[0][j] = mk(static_cast<CT>(x[rbase + 2 * k]), static_cast<CT>(x[rbase + 2 * k + 1]));
       }
-      if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+      if constexpr (Cfg::TWL) load_twiddles<N, NT>(tw_s, tw_g);
       if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
       __syncthreads();
     } else {
@@ -568,7 +571,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
         if (g0 == 0) {
           if constexpr (MODE == ROW_P1) red[1] = row_sum(j0);
           if constexpr (PF) load_phi(rbase, ph);
-          if constexpr (tw_in_lds<N, CT>()) load_twiddles<N, NT>(tw_s, tw_g);
+          if constexpr (Cfg::TWL) load_twiddles<N, NT>(tw_s, tw_g);
           if constexpr (ST) load_stage_tables<N, L, E, NT>(st_s, tw_g);
         }
         tile_to_lds(g0, tl);
@@ -647,6 +650,7 @@ struct ColCfg {
   static constexpr int AS = XM == 0 ? 0 : C * LP, BS = 2 * C * LP;
   static constexpr int LDS = (XM == 0 ? 1 : XM == 1 ? 2 : 4) * C * LP;
   static constexpr bool ST = SC::T == 1;
+  static constexpr bool TWL = tw_in_lds<N, CT>() || SC::T == 2;   // half table in LDS
   static_assert(!ST || stab_fits<N, CT, N, E>(), "stage tables do not fit this schedule");
   static constexpr int NT = C * TPL;
   static constexpr bool WAVE = TPL <= 64;
@@ -664,7 +668,7 @@ struct ColCfg {
 template <class Cfg, class CT>
 struct ColShared {
   static constexpr int NLDS = Cfg::LDS,
-                       NTW = Cfg::ST ? stab_len(Cfg::NN, Cfg::E) : (tw_in_lds<Cfg::NN, CT>() ? Cfg::NN / 2 : 0),
+                       NTW = Cfg::ST ? stab_len(Cfg::NN, Cfg::E) : (Cfg::TWL ? Cfg::NN / 2 : 0),
                        NSCR = 8 * ((Cfg::NT + 63) / 64);
   static constexpr int OFF_TW = NLDS * (int)sizeof(cplx<CT>);
   static constexpr int OFF_SCR = OFF_TW + NTW * (int)sizeof(cplx<CT>);

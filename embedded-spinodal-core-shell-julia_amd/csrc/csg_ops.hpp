@@ -96,7 +96,7 @@ void add_cand(PassCands& pc) {
     using Cfg = RowCfg<T, TIN, T, N, MODE, SC>;
     pc.fn[i] = launch_row<T, TIN, T, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0,
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : (SC::T == 2 ? 2 : 0),
                                  PASS == 3 ? (Cfg::SPLIT ? 1 : 0) : 0};
     for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   } else {
@@ -104,7 +104,7 @@ void add_cand(PassCands& pc) {
     using Cfg = ColCfg<T, double, N, MODE, SC>;
     pc.fn[i] = launch_col<T, double, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
-    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::C, 0, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : 0, 0};
+    const int r[SCHED_FIELDS] = {Cfg::E, Cfg::C, 0, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : (SC::T == 2 ? 2 : 0), 0};
     for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   }
 }

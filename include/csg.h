@@ -175,7 +175,8 @@ int csg_set_params(csg_ctx* ctx, const csg_params* p);
  * csg_schedule_info: count of candidates of `pass`; `sched` (may be NULL,
  * else 7 * count int32) receives each candidate's [elements per thread, rows
  * or lines per workgroup, tile-load group, prefetch, FFT exchange layout,
- * twiddle source (1 = per-stage tables), P3 split (1 = one gradient component
+ * twiddle source (1 = per-stage tables, 2 = half table in LDS beyond 16 KiB),
+ * P3 split (1 = one gradient component
  * per workgroup; 0 for the other passes)]. */
 int csg_schedule_info(const csg_ctx* ctx, int32_t pass, int32_t* count, int32_t* sched);
 /* Selected candidate per pass (sel[0..3]). */
