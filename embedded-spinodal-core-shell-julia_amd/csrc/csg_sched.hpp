@@ -99,12 +99,13 @@ template <> struct StepCands<2048, false, 2> { using type = SchedList<Sched<16, 
 template <> struct StepCands<2048, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 0>, Sched<16, 8, 1, 0, 0, 0>, Sched<8, 2, 1, 0, 0, 0>>; };  // 30.8 / 32.5 / 34.0 us
 template <> struct StepCands<2048, false, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 0>, Sched<16, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 1>>; };  // 23.1 / 24.5 / 27.5 us
 // 4096^2 FP64: the 32 KiB half table of W_4096 staged in LDS (T = 2) instead
-// of read through the caches in every FFT stage; candidates with and without
-// it ranked by their in-loop event times (A/B on one box)
-template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 1, 2, 2>, Sched<8, 1, 0, 1, 2, 0>, Sched<4, 1, 0, 1, 2, 0>, Sched<8, 2, 1, 1, 2, 0>>; };  // in the loop 262.6 / 271.3 / 282.5 / 317.2 us
+// of read through the caches in every FFT stage, with the (G, PF, X) that
+// then time best; candidates with and without it ranked by their in-loop
+// event times (A/B on one box)
+template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 0, 2, 2>, Sched<8, 1, 0, 1, 2, 2>, Sched<8, 1, 0, 1, 2, 0>, Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 0, 0, 2>>; };  // in the loop 248.4 / 261.5 / 269.6 / 282.7 / 248.0 us
 template <> struct StepCands<4096, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 2>, Sched<8, 1, 0, 1, 1, 0>, Sched<16, 1, 0, 1, 1, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // in the loop 181.2 / 183.7 / 195.5 / 284.9 us
-template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 0, 1, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<16, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>>; };  // in the loop 333.6 / 349.8 / 418.6 / 379.0 us
-template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 0>>; };  // in the loop 169.0 / 189.8 / 185.8 / 226.2 us
+template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 1, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>, Sched<8, 2, 1, 0, 0, 2>>; };  // in the loop 325.5 / 332.2 / 349.4 / 379.8 / 339.2 us
+template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 0>>; };  // in the loop 169.0 / 173.1 / 187.8 / 185.1 / 226.6 us
 template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<8, 2, 0, 0, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 0, 2, 1>>; };  // 115.8 / 138.6 / 138.8 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
 template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // 161.9 / 174.0 / 186.2 us
