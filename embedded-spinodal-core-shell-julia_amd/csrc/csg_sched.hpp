@@ -81,14 +81,16 @@ template <> struct StepCands<768, false, 3> { using type = SchedList<Sched<6, 8,
 template <> struct StepCands<768, false, 4> { using type = SchedList<Sched<12, 2, 0, 0, 0, 0>, Sched<12, 1, 0, 0, 0, 0>, Sched<6, 1, 0, 0, 2, 0>>; };  // 9.5 / 9.7 / 9.8 us
 // 1024^2 FP64 P1: ranked by the captured loop's step time, where two rows
 // of two waves each beat four one-wave rows (50.2 vs 50.8 us per step)
-// although they lose alone (isolated 11.2 us for Sched<8, 4, ...>)
-template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 2, 0>>; };
+// although they lose alone (isolated 11.2 us for Sched<8, 4, ...>); with
+// the LDS-only exchange barrier, issuing the whole tile at once (G = E)
+// is faster again: in the loop 13.4 vs 13.9 us, step 50.2 vs 51.0 us
+template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // in the loop 13.4 / 13.9 / 14.0 / 16.4 us
 template <> struct StepCands<1024, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<16, 2, 0, 1, 0, 0>>; };  // 13.1 / 14.1 / 14.8 us
 template <> struct StepCands<1024, true, 3> { using type = SchedList<Sched<8, 8, 1, 0, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 0, 0>, Sched<8, 4, 1, 0, 0, 1, 0>>; };  // 14.6 / 14.9 / 16.6 us; in the loop 14.0 / 15.5 / 19.9, and 17.1 for the last (both components, phi read once)
 template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // 10.4 / 11.1 / 12.4 us
 template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4, 1, 0, 0, 0>, Sched<8, 2, 0, 0, 0, 0>, Sched<4, 2, 1, 0, 2, 1>>; };  // 8.6 / 9.4 / 9.6 us
 template <> struct StepCands<1024, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 1, 0, 0>>; };  // 8.6 / 9.4 / 10.7 us
-template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // 8.5 / 8.8 / 10.6 us; in the loop 11.3 / 11.9 / 13.8, and 14.1 for the last (both components, phi read once)
+template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 0, 0, 2, 1>, Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // in the loop 11.3 / 11.5 / 11.9 / 13.9 us, and 14.3 for the last (both components, phi read once)
 template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // 7.7 / 8.8 / 10.5 us
 template <> struct StepCands<2048, true, 1> { using type = SchedList<Sched<4, 1, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // 49.6 / 51.0 / 56.9 us
 template <> struct StepCands<2048, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 0>, Sched<16, 1, 0, 1, 0, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // 44.8 / 51.4 / 52.9 us
@@ -108,7 +110,7 @@ template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2,
 template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 0>>; };  // in the loop 169.0 / 173.1 / 187.8 / 185.1 / 226.6 us
 template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<8, 2, 0, 0, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 0, 2, 1>>; };  // 115.8 / 138.6 / 138.8 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
-template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // 161.9 / 174.0 / 186.2 us
+template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 1>, Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // in the loop 132.3 / 137.3 / 159.0 / 174.3 us
 template <> struct StepCands<4096, false, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 1>, Sched<16, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 1>>; };  // 80.0 / 84.8 / 121.9 us
 
 }  // namespace csg
