@@ -3,7 +3,8 @@
 CPU: the oracle's adjoint (oracle/adjoint_oracle.py) against central finite
 differences of the oracle's own forward loss.
 GPU: the HIP adjoint (csg_adjoint_forward / csg_adjoint_backward) against the
-oracle adjoint (same inputs, tight tolerance) and against central finite
+oracle adjoint (same inputs, tight tolerance; 64^2 x 200 steps and the
+workload's 512^2 grid over 40 steps) and against central finite
 differences of the GPU forward loss, including the 512^2, 2000-step,
 45-step-checkpoint workload on 32 random directions.
 Loss: L = 1/2 ||c(T) - c*||^2 + lam F(c(T)), c* = forward run with kappa, P,
@@ -72,6 +73,22 @@ def test_gpu_adjoint_matches_oracle_adjoint(pkg):
     c0, phi, loss = _setup_oracle(n, 16.0, steps, dt, 0.1)
     L, g, dth = ad.gradient(c0, phi, P, dt, steps, loss)
     r = pkg.adjoint.gradient(c0, phi, loss.target, dt=dt, steps=steps, lam=0.1, ckpt_every=14)
+    assert abs(r.loss / L - 1) < 1e-12
+    assert np.max(np.abs(r.dc0 - g)) <= 1e-10 * np.max(np.abs(g))
+    for got, ref in zip((r.dkappa, r.dP, r.dchi), dth):
+        assert abs(got - ref) <= 1e-9 * abs(ref)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_oracle_adjoint_512(pkg):
+    """The adjoint workload's grid and particle (512^2, R=160, w=2, lam =
+    1e-3) against the CPU adjoint of the reference step, over 40 steps with
+    7-step checkpoints (segments recomputed from their checkpoints)."""
+    say("adjoint: GPU vs oracle, 512^2, 40 steps")
+    n, steps, dt = 512, 40, 0.01
+    c0, phi, loss = _setup_oracle(n, 160.0, steps, dt, 1e-3)
+    L, g, dth = ad.gradient(c0, phi, P, dt, steps, loss)
+    r = pkg.adjoint.gradient(c0, phi, loss.target, dt=dt, steps=steps, lam=1e-3, ckpt_every=7)
     assert abs(r.loss / L - 1) < 1e-12
     assert np.max(np.abs(r.dc0 - g)) <= 1e-10 * np.max(np.abs(g))
     for got, ref in zip((r.dkappa, r.dP, r.dchi), dth):
