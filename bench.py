@@ -183,10 +183,16 @@ def _rocprof_us(kernel: str, workload: str, form: str, sched: str):
     return None
 
 
+_HIP = []
+
+
 def _device_sync():
-    """hipDeviceSynchronize: the barrier on both sides of a timed region."""
-    import ctypes
-    if ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize() != 0:
+    """hipDeviceSynchronize: the barrier on both sides of a timed region (the
+    runtime library is loaded once, outside the timed regions)."""
+    if not _HIP:
+        import ctypes
+        _HIP.append(ctypes.CDLL("libamdhip64.so"))
+    if _HIP[0].hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")
 
 

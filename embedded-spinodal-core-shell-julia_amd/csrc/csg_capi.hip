@@ -49,7 +49,8 @@ struct csg_ctx {
   long long* ctr = nullptr;
   double* hist = nullptr;
   size_t hist_cap = 0;    // rows
-  std::vector<long long> hstage;   // host staging of ctr (2 words) + history rows
+  long long* hstage = nullptr;     // pinned host staging of ctr (2 words) + history rows
+  size_t hstage_cap = 0;           // 8-byte words
   // non-finite checks during long runs: pinned slots for ctr[1] and events
   long long* flag_pin = nullptr;
   hipEvent_t flag_ev[2] = {nullptr, nullptr};
@@ -57,7 +58,7 @@ struct csg_ctx {
   double* rate = nullptr; // (N/2+1) x N complex doubles
   bool fields_set = false;
   long long nonfinite = -1;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;   // STEPS_PER_GRAPH loop iterations of csg_run
   double gdt = 0.0;
   int64_t gevery = 1;     // history interval captured in gexec
   DevParams gp{};         // parameters captured in gexec
@@ -210,12 +211,26 @@ int set_state_from_real(csg_ctx* c, void* S) {
   return CSG_OK;
 }
 
+// Destroys the captured loop of csg_run (after a change of the kernels, the
+// history buffer or the loop's arguments).
+int drop_step_graphs(csg_ctx* c) {
+  if (c->gexec) {
+    const hipError_t e = hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    if (e != hipSuccess) return fail(CSG_ERR_HIP, std::string("hipGraphExecDestroy: ") + hipGetErrorString(e));
+  }
+  return CSG_OK;
+}
+
+// History rows on the device: at least HIST_MIN_ROWS, so that short runs of
+// different lengths keep one buffer (and the loops captured with it).
+constexpr size_t HIST_MIN_ROWS = 1024;
+
 int ensure_hist(csg_ctx* c, size_t rows) {
   if (rows <= c->hist_cap) return CSG_OK;
-  if (c->gexec) {  // the captured graph holds the old history pointer
-    HIPCHK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
+  rows = std::max(rows, HIST_MIN_ROWS);
+  // the captured loops hold the old history pointer
+  if (int rc = drop_step_graphs(c)) return rc;
   if (c->hist) {
     HIPCHK(hipFree(c->hist));
     c->bytes -= c->hist_cap * 6 * sizeof(double);
@@ -228,21 +243,24 @@ int ensure_hist(csg_ctx* c, size_t rows) {
   return CSG_OK;
 }
 
+__global__ void __launch_bounds__(64) reset_ctr_kernel(long long* ctr) {
+  if (threadIdx.x < 2) ctr[threadIdx.x] = threadIdx.x == 0 ? 0 : -1;
+}
+
+// Step counter 0, no non-finite step yet (a kernel: no host copy to wait for).
 int reset_ctr(csg_ctx* c) {
-  // static storage: the asynchronous copy may read the source after return
-  static const long long init[2] = {0, -1};
-  HIPCHK(hipMemcpyAsync(c->ctr, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(reset_ctr_kernel, dim3(1), dim3(64), 0, c->stream, c->ctr);
+  HIPCHK(hipGetLastError());
   return CSG_OK;
 }
 
 bool same_params(const DevParams& x, const DevParams& y) { return std::memcmp(&x, &y, sizeof(DevParams)) == 0; }
 
+// The captured block of STEPS_PER_GRAPH loop iterations for (dt, every) and
+// the current parameters.
 int build_graph(csg_ctx* c, double dt, int64_t every) {
   if (c->gexec && c->gdt == dt && c->gevery == every && same_params(c->gp, c->dp)) return CSG_OK;
-  if (c->gexec) {
-    HIPCHK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
+  if (int rc = drop_step_graphs(c)) return rc;
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_iteration(c, dt, true, every);
@@ -272,9 +290,21 @@ int ensure_flag_slots(csg_ctx* c) {
   return CSG_OK;
 }
 
-void ensure_hstage(csg_ctx* c, size_t rows) {
-  const size_t need = 2 + rows * 6;   // 8-byte words
-  if (c->hstage.size() < need) c->hstage.resize(need);
+// Pinned host staging for ctr and the history rows of a run (at least
+// HIST_MIN_ROWS rows), so the copies back are direct transfers.
+int ensure_hstage(csg_ctx* c, size_t rows) {
+  const size_t need = 2 + std::max(rows, HIST_MIN_ROWS) * 6;   // 8-byte words
+  if (c->hstage_cap >= need) return CSG_OK;
+  if (c->hstage) {
+    HIPCHK(hipHostFree(c->hstage));
+    c->hstage = nullptr;
+    c->hstage_cap = 0;
+  }
+  void* p = nullptr;
+  HIPCHK(hipHostMalloc(&p, need * sizeof(long long), hipHostMallocDefault));
+  c->hstage = static_cast<long long*>(p);
+  c->hstage_cap = need;
+  return CSG_OK;
 }
 
 int check_ctx(const csg_ctx* c) {
@@ -362,7 +392,7 @@ int csg_destroy(csg_ctx* c) {
   if (!c) return CSG_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  (void)drop_step_graphs(c);
   drop_adjoint_graphs(c);
   void* bufs[] = {c->S, c->S2, c->A, c->Bb, c->phi, c->real, c->part, c->ctr, c->hist,
                   c->scratch_row, c->rate, c->tw64, c->G, c->Xr, c->adj_acc};
@@ -374,6 +404,7 @@ int csg_destroy(csg_ctx* c) {
     if (b) (void)hipFree(b);
   if (c->tw && c->tw != c->tw64) (void)hipFree(c->tw);
   if (c->flag_pin) (void)hipHostFree(c->flag_pin);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   for (hipEvent_t ev : c->flag_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -425,10 +456,15 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
   // the device keeps rows s = 0, hist_every, 2 hist_every, ... only
   const int64_t rows = steps / hist_every + 1;
   if (int rc = ensure_hist(c, static_cast<size_t>(rows))) return rc;
-  ensure_hstage(c, history ? static_cast<size_t>(rows) : 0);
+  if (int rc = ensure_hstage(c, history ? static_cast<size_t>(rows) : 0)) return rc;
   if (int rc = reset_ctr(c)) return rc;
   int64_t done = 0;
   bool stopped = false;
+  // Long runs replay a captured block of STEPS_PER_GRAPH iterations; the
+  // remainder and the last, history-only iteration are launched directly.
+  // (Replays are no faster than direct launches on MI355X, and the first
+  // kernel of a directly launched run starts sooner; the graph saves host
+  // work on long runs.)
   if (steps >= 2 * STEPS_PER_GRAPH) {
     if (int rc = build_graph(c, dt, hist_every)) return rc;
     // Long runs check the device's first-non-finite step between blocks of
@@ -465,8 +501,8 @@ int csg_run(csg_ctx* c, double dt, int64_t steps, int64_t hist_every, double* hi
     enqueue_iteration(c, dt, false, hist_every);
   }
   HIPCHK(hipGetLastError());
-  long long* ctr = c->hstage.data();
-  double* h = reinterpret_cast<double*>(c->hstage.data() + 2);
+  long long* ctr = c->hstage;
+  double* h = reinterpret_cast<double*>(c->hstage + 2);
   HIPCHK(hipMemcpyAsync(ctr, c->ctr, 2 * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
   if (history)
     HIPCHK(hipMemcpyAsync(h, c->hist, static_cast<size_t>(rows) * 6 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1150,10 +1186,7 @@ constexpr int TUNE_REPS = 4;   // timed graph replays (32 loop iterations each) 
 // sel[0..3]; the history counter is reset so the rows stay in the buffer.
 int time_schedule(csg_ctx* c, double dt, const int sel[4], hipEvent_t e0, hipEvent_t e1, double* us) {
   select_schedule(c->ops, sel);
-  if (c->gexec) {
-    HIPCHK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
+  if (int rc = drop_step_graphs(c)) return rc;
   if (int rc = build_graph(c, dt, 1)) return rc;
   if (int rc = reset_ctr(c)) return rc;
   HIPCHK(hipGraphLaunch(c->gexec, c->stream));   // warm-up
@@ -1197,10 +1230,8 @@ int csg_set_schedule(csg_ctx* c, const int32_t* sel) {
   if (same) return CSG_OK;
   if (!select_schedule(c->ops, s)) return fail(CSG_ERR_ARG, "schedule index out of range (csg_schedule_info)");
   HIPCHK(hipSetDevice(c->device));
-  if (c->gexec) {   // the captured loops hold the previous kernels
-    HIPCHK(hipGraphExecDestroy(c->gexec));
-    c->gexec = nullptr;
-  }
+  // the captured loops hold the previous kernels
+  if (int rc = drop_step_graphs(c)) return rc;
   drop_adjoint_graphs(c);
   return CSG_OK;
 }
@@ -1242,10 +1273,7 @@ int csg_autotune(csg_ctx* c, double dt, int32_t rounds, int32_t* sel_out, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   select_schedule(c->ops, best);
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  (void)drop_step_graphs(c);
   drop_adjoint_graphs(c);
   HIPCHK(hipMemcpyAsync(c->S, c->S2, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

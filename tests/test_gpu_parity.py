@@ -245,8 +245,10 @@ def test_history_every_and_run_shape(pkg):
 
 
 def test_graph_replays_match_eager(pkg):
-    """Runs long enough to use the captured graph (>= 64 steps) agree with
-    eager per-step launches, replay after replay."""
+    """Runs of >= 64 steps replay captured blocks of 32 iterations and launch
+    the remainder directly; every run length agrees bit for bit with eager
+    per-step launches, replay after replay, and its history rows with those
+    of a longer run."""
     say("graph vs eager")
     c0, phi = _fields(pkg, 128)
     s = pkg.Session(128)
@@ -254,8 +256,19 @@ def test_graph_replays_match_eager(pkg):
     s.step(0.01, 200)                       # eager launches
     eager = s.field()
     for _ in range(3):
-        cg, _ = pkg.run_simulation(c0, phi, steps=200)   # 6 graph replays + 8 eager
+        cg, _ = pkg.run_simulation(c0, phi, steps=200)   # 6 replays + 8 direct
         assert np.array_equal(cg, eager)
+    _, h_all = pkg.run_simulation(c0, phi, steps=200)
+    for k in (0, 1, 5, 20, 31, 33, 63, 64, 95):
+        say(f"graph vs eager: {k} steps")
+        s.set_fields(c0, phi)
+        s.step(0.01, k)
+        ek = s.field()
+        s.set_fields(c0, phi)
+        hk = s.run(0.01, k)
+        assert np.array_equal(s.field(), ek), k
+        assert np.array_equal(hk, h_all[: k + 1]), k
+    s.close()
 
 
 def test_nonfinite_at_step0(pkg):
