@@ -104,7 +104,6 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const int kx = blockIdx.x * Cfg::C + l;
   cplx<double>* line = lds + l * LP;
-  const size_t base = (size_t)kx * N;
   const DevParams& p = a.p;
   const double inv_nsp = p.inv_nsp, dt = a.dt;
   const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);
@@ -113,7 +112,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   cplx<double> v[2][E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const size_t o = base + t + TPL * j;
+    const size_t o = lidx<N>(kx, t + TPL * j);
     v[0][j] = a.G[o];
     v[1][j] = a.S1[o] - a.S0[o];
   }
@@ -155,8 +154,8 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    a.X0[base + t + TPL * j] = v[0][j];
-    a.X1[base + t + TPL * j] = v[1][j];
+    a.X0[lidx<N>(kx, t + TPL * j)] = v[0][j];
+    a.X1[lidx<N>(kx, t + TPL * j)] = v[1][j];
   }
   double r[1] = {acc};
   const int ops[1] = {RED_SUM};
@@ -177,7 +176,6 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   const int l = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const int kx = blockIdx.x * Cfg::C + l;
   cplx<double>* line = lds + l * LP;
-  const size_t base = (size_t)kx * N;
   const DevParams& p = a.p;
   const double inv_nsp = p.inv_nsp, dt = a.dt;
   const double kN2 = kval<N>(N / 2, inv_nsp) * kval<N>(N / 2, inv_nsp);
@@ -186,7 +184,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   cplx<double> v[3][E], gp[E];
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const size_t o = base + t + TPL * j;
+    const size_t o = lidx<N>(kx, t + TPL * j);
     v[0][j] = a.X0[o];
     v[1][j] = a.X1[o];
     v[2][j] = a.S0[o];
@@ -232,7 +230,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const size_t o = base + t + TPL * j;
+    const size_t o = lidx<N>(kx, t + TPL * j);
     a.Xr[o] = w[0][j];
     a.G[o] = gp[j] + w[1][j];
   }
@@ -293,8 +291,9 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   double ph[E][2];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
-    const int f = threadIdx.x + NT * r;
-    const size_t o = (size_t)(f / B) * N + j0 + f % B;
+    int bb, kx;
+    tile_elem<N, B>(threadIdx.x + NT * r, bb, kx);
+    const size_t o = lidx<N>(kx, j0 + bb);
     tr_x[r] = a.Xr[o];
     tr_s[r] = a.S0[o];
     gp[r] = a.G[o];
@@ -308,9 +307,10 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   if constexpr (tw_in_lds<N, double>()) load_twiddles<N, NT>(tw_s, tw_g);
 #pragma unroll
   for (int r = 0; r < E; ++r) {
-    const int f = threadIdx.x + NT * r;
-    lds[(f % B) * LP + pad<double, L>(f / B)] = tr_x[r];
-    lds[(B + f % B) * LP + pad<double, L>(f / B)] = tr_s[r];
+    int bb, kx;
+    tile_elem<N, B>(threadIdx.x + NT * r, bb, kx);
+    lds[bb * LP + pad<double, L>(kx)] = tr_x[r];
+    lds[(B + bb) * LP + pad<double, L>(kx)] = tr_s[r];
   }
   __syncthreads();
   cplx<double> v[2][E];
@@ -374,9 +374,9 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < E; ++r) {
-    const int f = threadIdx.x + NT * r;
-    const int bb = f % B, kx = f / B;
-    a.G[(size_t)kx * N + j0 + bb] = gp[r] + lds[bb * LP + pad<double, L>(kx)];
+    int bb, kx;
+    tile_elem<N, B>(threadIdx.x + NT * r, bb, kx);
+    a.G[lidx<N>(kx, j0 + bb)] = gp[r] + lds[bb * LP + pad<double, L>(kx)];
   }
   double rr[1] = {srg};
   const int ops[1] = {RED_SUM};

@@ -135,6 +135,38 @@ __device__ __forceinline__ PassArgs instance_view(const PassArgs& a0, int q) {
 
 template <class T> __device__ __forceinline__ bool finite(T v) { return isfinite(v); }
 
+// Line blocking of the line layout (a build option, off by default).  The L
+// lines are stored in blocks of KB consecutive k_x, j-major inside a block:
+// element (kx, j) sits at (kx / KB) N KB + j KB + kx % KB.  A row tile of b
+// rows then moves b KB contiguous elements per block instead of b, and a
+// column workgroup reads its line with a stride of KB elements.  Measured at
+// 4096^2 with KB = 2 (-DCSG_LINE_BLOCK_MIN_N=4096): the row passes gain 6-8 %
+// (FP64) and 4-6 % (mixed), the column passes lose 40-70 %, so KB = 1 (the
+// plain layout kx N + j) is used for every grid.
+#ifndef CSG_LINE_BLOCK_MIN_N
+#define CSG_LINE_BLOCK_MIN_N (1 << 30)
+#endif
+#ifndef CSG_LINE_BLOCK
+#define CSG_LINE_BLOCK 2
+#endif
+template <int N> constexpr int line_block() { return N >= CSG_LINE_BLOCK_MIN_N ? CSG_LINE_BLOCK : 1; }
+
+template <int N, int KB = line_block<N>()> __host__ __device__ __forceinline__ size_t lidx(int kx, int j) {
+  if constexpr (KB == 1) return (size_t)kx * N + j;
+  else return (size_t)(kx / KB) * (N * KB) + (size_t)j * KB + (kx % KB);
+}
+
+// Element f of a row tile of B rows in its transposed (line-layout) order:
+// lanes take consecutive k_x of a block, then consecutive rows, then blocks,
+// so neighbouring lanes touch neighbouring addresses.  Sets the tile row bb
+// and the k_x.
+template <int N, int B, int KB = line_block<N>()>
+__host__ __device__ __forceinline__ void tile_elem(int f, int& bb, int& kx) {
+  const int r = f / KB;
+  bb = r % B;
+  kx = (r / B) * KB + f % KB;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Remap so that each XCD receives a contiguous range of tiles: row
 // tiles j0 and j0+B share 128-byte lines of the line-layout arrays, which
@@ -468,8 +500,9 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
       const cplx<TIN>* X = static_cast<const cplx<TIN>*>((arr0 + q) == 0 ? a.in0 : a.in1);
 #pragma unroll
       for (int r = 0; r < G; ++r) {
-        const int f = threadIdx.x + NT * (g0 + r);
-        tl[q][g0 + r] = X[(size_t)(f / B) * N + j0 + f % B];
+        int bb, kx;
+        tile_elem<N, B>(threadIdx.x + NT * (g0 + r), bb, kx);
+        tl[q][g0 + r] = X[lidx<N>(kx, j0 + bb)];
       }
     }
   };
@@ -478,13 +511,15 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
     for (int q = 0; q < NARR; ++q)
 #pragma unroll
       for (int r = 0; r < G; ++r) {
-        const int f = threadIdx.x + NT * (g0 + r);
-        lds[(q * B + f % B) * LP + pad<CT, L>(f / B)] = cvt<CT>(tl[q][g0 + r]);
+        int bb, kx;
+        tile_elem<N, B>(threadIdx.x + NT * (g0 + r), bb, kx);
+        lds[(q * B + bb) * LP + pad<CT, L>(kx)] = cvt<CT>(tl[q][g0 + r]);
       }
   };
   // mean composition from the FP64 k_x = 0 row sums of the state (P1)
   auto row_sum = [&](int j0) {
-    return threadIdx.x < B ? static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[j0 + threadIdx.x].x) : 0.0;
+    return threadIdx.x < B ? static_cast<double>(static_cast<const cplx<TIN>*>(a.in0)[lidx<N>(0, j0 + threadIdx.x)].x)
+                           : 0.0;
   };
   // C2R pre-processing fused with the read: Z_k = (X_k + conj X_{L-k}) +
   // i (X_k - conj X_{L-k}) W_N^{-k}, times 1/N; slot 0 holds (X_0, X_L).
@@ -523,7 +558,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
       // one row: the thread's own elements k = t + TPL j are its stores
       cplx<TOUT>* X = static_cast<cplx<TOUT>*>(arr0 == 0 ? a.out0 : a.out1);
 #pragma unroll
-      for (int j = 0; j < E; ++j) store_out(&X[(size_t)(t + TPL * j) * N + j0], cvt<TOUT>(v[0][j]), a.nt);
+      for (int j = 0; j < E; ++j) store_out(&X[lidx<N>(t + TPL * j, j0)], cvt<TOUT>(v[0][j]), a.nt);
       return;
     }
 #pragma unroll
@@ -538,9 +573,9 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
       cplx<TOUT>* X = static_cast<cplx<TOUT>*>((arr0 + q) == 0 ? a.out0 : a.out1);
 #pragma unroll
       for (int r = 0; r < E; ++r) {
-        const int f = threadIdx.x + NT * r;
-        const int bb = f % B, kx = f / B;
-        store_out(&X[(size_t)kx * N + j0 + bb], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT, L>(kx)]), a.nt);
+        int bb, kx;
+        tile_elem<N, B>(threadIdx.x + NT * r, bb, kx);
+        store_out(&X[lidx<N>(kx, j0 + bb)], cvt<TOUT>(lds[(q * B + bb) * LP + pad<CT, L>(kx)]), a.nt);
       }
     }
   };

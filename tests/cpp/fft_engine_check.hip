@@ -10,7 +10,7 @@
 #include <cstdio>
 #include <vector>
 
-#include "../../embedded-spinodal-core-shell-julia_amd/csrc/csg_fft.hpp"
+#include "../../embedded-spinodal-core-shell-julia_amd/csrc/csg_kernels.hpp"
 
 using namespace csg;
 
@@ -153,8 +153,43 @@ int check_slots() {
 template <int LEN>
 int check_slots_both() { return check_slots<double, LEN>() + check_slots<float, LEN>(); }
 
+// The line-layout map of a grid of N (blocks of KB lines) and the element
+// order of row tiles of B rows: every tile element (row bb, line kx) is
+// visited once, the tiles of a grid cover the N*L elements once, and the B*KB
+// elements of one block and tile sit at consecutive addresses in lane order.
+template <int N, int B, int KB>
+int check_line_map() {
+  constexpr int L = N / 2;
+  std::vector<char> seen(static_cast<size_t>(N) * L, 0);
+  for (int j0 = 0; j0 < N; j0 += B) {
+    for (int f = 0; f < B * L; ++f) {
+      int bb, kx;
+      tile_elem<N, B, KB>(f, bb, kx);
+      const size_t o = lidx<N, KB>(kx, j0 + bb);
+      if (bb < 0 || bb >= B || kx < 0 || kx >= L || o >= seen.size() || seen[o]) {
+        printf("N=%d B=%d KB=%d: line map not a bijection at j0=%d f=%d\n", N, B, KB, j0, f);
+        return 1;
+      }
+      seen[o] = 1;
+      if (f % (B * KB) != 0) {
+        int pb, pk;
+        tile_elem<N, B, KB>(f - 1, pb, pk);
+        if (o != lidx<N, KB>(pk, j0 + pb) + 1) {
+          printf("N=%d B=%d KB=%d: tile block not contiguous at f=%d\n", N, B, KB, f);
+          return 1;
+        }
+      }
+    }
+  }
+  if (KB == 1 && lidx<N, KB>(3, 5) != static_cast<size_t>(3) * N + 5) { printf("KB=1 is not kx N + j\n"); return 1; }
+  return 0;
+}
+
 int main() {
   int bad = 0;
+  bad += check_line_map<64, 1, 1>() + check_line_map<64, 4, 1>() + check_line_map<64, 1, 2>() +
+         check_line_map<64, 2, 2>() + check_line_map<64, 4, 4>() + check_line_map<256, 8, 2>() +
+         check_line_map<4096, 1, 2>() + check_line_map<4096, 2, 2>() + check_line_map<4096, 2, 1>();
   for (int n : {16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 48, 96, 192, 384, 768}) bad += check_layout(n);
   bad += check_slots_both<8>() + check_slots_both<16>() + check_slots_both<24>() + check_slots_both<32>() +
          check_slots_both<48>() + check_slots_both<64>() + check_slots_both<96>() + check_slots_both<128>() +

@@ -113,9 +113,10 @@ def test_fft_engine_host_check(tmp_path):
     """The register Stockham engine's stage code (plans, radix-16/8/4/2
     butterflies, swizzled or padded exchanges, twiddles from the half table
     and from the per-stage tables) against a long-double DFT for every line
-    length 8..4096, FP64 and FP32; the twiddle-buffer layout; and the LDS
-    slot map (injective into the buffer) and tile row strides -- compiled for
-    the host."""
+    length 8..4096, FP64 and FP32; the twiddle-buffer layout; the LDS slot
+    map (injective into the buffer) and tile row strides; and the line-layout
+    map with its row-tile element order (a bijection, contiguous per block,
+    for line blocks of 1, 2 and 4) -- compiled for the host."""
     import subprocess
     exe = tmp_path / "fft_engine_check"
     src = os.path.join(ROOT, "tests", "cpp", "fft_engine_check.hip")
