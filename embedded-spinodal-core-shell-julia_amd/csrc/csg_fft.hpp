@@ -339,9 +339,11 @@ CSG_HD cplx<T> tw_at(const cplx<T>* tw, int e) {
   return w;
 }
 
-// Stage the half table in LDS: all loads issued, then all LDS stores.  The
-// loads are unconditional (index clamped into the table) so the staging
-// values stay in registers; only the stores are predicated.
+// Stage the half table in LDS: all loads issued, then all LDS stores.  Loads
+// and stores are unconditional, with the index clamped into the table (lanes
+// past its end store the last entry's own value again): a predicated store
+// becomes a branch that the compiler fills with its load and a wait for every
+// load issued before it, which serialises the kernel's first memory accesses.
 template <int NTAB, int NT, class T>
 __device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __restrict__ g) {
   constexpr int H = NTAB / 2, K = (H + NT - 1) / NT;
@@ -354,13 +356,13 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* lds_tw, const cplx<T>* __
 #pragma unroll
   for (int r = 0; r < K; ++r) {
     const int m = static_cast<int>(threadIdx.x) + NT * r;
-    if (m < H) lds_tw[m] = w[r];
+    lds_tw[m < H ? m : H - 1] = w[r];
   }
 }
 
 // Copy the stage table(s) of the (LEN, E) plans of an N-point grid's
-// twiddle buffer g (twiddle_len(N) entries) to LDS: all loads issued
-// (clamped), then the LDS stores, as in load_twiddles.
+// twiddle buffer g (twiddle_len(N) entries) to LDS: all loads issued, then
+// the LDS stores, both clamped and unconditional, as in load_twiddles.
 template <int N, int LEN, int E, int NT, class T>
 __device__ __forceinline__ void load_stage_tables(cplx<T>* st, const cplx<T>* __restrict__ g) {
   constexpr int Q = stab_len(LEN, E), K = (Q + NT - 1) / NT, OFF = stab_offset(N, LEN, E);
@@ -374,7 +376,7 @@ __device__ __forceinline__ void load_stage_tables(cplx<T>* st, const cplx<T>* __
 #pragma unroll
   for (int r = 0; r < K; ++r) {
     const int q = static_cast<int>(threadIdx.x) + NT * r;
-    if (q < Q) st[q] = w[r];
+    st[q < Q ? q : Q - 1] = w[r];
   }
 }
 

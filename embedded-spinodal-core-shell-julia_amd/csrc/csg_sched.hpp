@@ -84,14 +84,14 @@ template <> struct StepCands<768, false, 4> { using type = SchedList<Sched<12, 2
 // although they lose alone (isolated 11.2 us for Sched<8, 4, ...>); with
 // the LDS-only exchange barrier, issuing the whole tile at once (G = E)
 // is faster again: in the loop 13.4 vs 13.9 us, step 50.2 vs 51.0 us
-template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // in the loop 13.4 / 13.9 / 14.0 / 16.4 us
+template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 0, 1, 0, 1>, Sched<4, 2, 0, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 4, 0, 0, 2, 1>>; };  // in the loop, after the branch-free twiddle staging: 48.7 / 48.8 / 49.6 / 49.0 / 48.9 us per step (the first against the third, alternating: 47.7 vs 48.4)
 template <> struct StepCands<1024, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<16, 2, 0, 1, 0, 0>>; };  // 13.1 / 14.1 / 14.8 us
 template <> struct StepCands<1024, true, 3> { using type = SchedList<Sched<8, 8, 1, 0, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 0, 0>, Sched<8, 4, 1, 0, 0, 1, 0>>; };  // 14.6 / 14.9 / 16.6 us; in the loop 14.0 / 15.5 / 19.9, and 17.1 for the last (both components, phi read once)
 template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // 10.4 / 11.1 / 12.4 us
-template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4, 1, 0, 0, 0>, Sched<8, 2, 0, 0, 0, 0>, Sched<4, 2, 1, 0, 2, 1>>; };  // 8.6 / 9.4 / 9.6 us
+template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4, 1, 0, 0, 0>, Sched<4, 2, 0, 0, 2, 1>, Sched<4, 2, 1, 1, 2, 1>, Sched<8, 4, 1, 1, 0, 1>, Sched<8, 2, 0, 0, 0, 0>>; };  // 8.6 / - / - / - / 9.4 us; in the loop after the branch-free twiddle staging the first four are within 0.1 us per step of each other (36.8-36.9 us with P3 candidate 1)
 template <> struct StepCands<1024, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 1, 0, 0>>; };  // 8.6 / 9.4 / 10.7 us
-template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 0, 0, 2, 1>, Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // in the loop 11.3 / 11.5 / 11.9 / 13.9 us, and 14.3 for the last (both components, phi read once)
-template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // 7.7 / 8.8 / 10.5 us
+template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 0, 0, 2, 1>, Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // in the loop 11.3 / 11.5 / 11.9 / 13.9 us, and 14.3 for the last (both components, phi read once); the first two equal after the branch-free twiddle staging
+template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // 7.7 / - / 8.8 / 10.5 us; in the loop after the branch-free twiddle staging 37.5 / 37.3 us per step
 template <> struct StepCands<2048, true, 1> { using type = SchedList<Sched<4, 1, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // 49.6 / 51.0 / 56.9 us
 template <> struct StepCands<2048, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 0>, Sched<16, 1, 0, 1, 0, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // 44.8 / 51.4 / 52.9 us
 template <> struct StepCands<2048, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 0>, Sched<16, 4, 1, 0, 0, 0>, Sched<4, 1, 1, 0, 0, 1>>; };  // 60.0 / 67.0 / 72.3 us
