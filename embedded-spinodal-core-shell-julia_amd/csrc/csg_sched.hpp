@@ -63,7 +63,7 @@ template <> struct StepCands<384, false, 1> { using type = SchedList<Sched<6, 2,
 template <> struct StepCands<384, false, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 0, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>>; };  // 6.6 / 7.2 / 8.7 / 11.4 us
 template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0, 0, 0>, Sched<6, 2, 1, 1, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>>; };  // 6.0 / 6.2 / 6.4 / 6.9 us
 template <> struct StepCands<384, false, 4> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 0, 0, 0>, Sched<6, 4, 0, 0, 1, 0>, Sched<12, 2, 0, 0, 0, 0>>; };  // 6.1 / 6.6 / 7.6 / 8.6 us
-template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 0>, Sched<4, 1, 1, 0, 2, 0>, Sched<4, 4, 0, 1, 2, 1>, Sched<4, 4, 1, 0, 2, 1>, Sched<4, 4, 0, 1, 0, 1>>; };  // 6.2 / 6.4 / 6.8 / 6.8; 64 instances per launch: 128.6 us
+template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 1, 1, 0, 2, 0>, Sched<4, 4, 0, 1, 2, 1>, Sched<4, 4, 0, 1, 0, 1>>; };  // in the loop after the branch-free twiddle staging: 27.3 / 28.5 us per step for the first two; earlier 6.2 / 6.4 / 6.8 us alone; 64 instances per launch: 128.6 us
 template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 2, 1>, Sched<8, 4, 0, 1, 1, 0>>; };  // 6.4 / 6.6 / 6.8 / 8.2; 64 instances per launch: 108.2 us
 template <> struct StepCands<512, true, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 0>, Sched<8, 8, 0, 0, 0, 0>>; };  // 5.9 / 6.6 / 6.6 / 7.4 us
 template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<8, 2, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 1, 1>, Sched<8, 2, 0, 1, 0, 0>>; };  // 5.7 / 6.1 / 6.2 / 7.2; 64 instances per launch: 94.8 us
