@@ -292,7 +292,8 @@ constexpr int row_rows() {
 //   G   row passes: transposed-tile loads issued in groups of G, each group
 //       written to LDS before the next is issued (E = all at once),
 //   PF  1: the pass's other inputs (phi in P1/P3, the state in P4) are loaded
-//       with the first group of loads; 0: where they are used,
+//       with the first group of loads; 0: where they are used; 2: once the
+//       first loads have arrived, before the first transform,
 //   X   FFT exchange layout (fft_regs XM: 0 one array at a time, 1 all
 //       arrays together, 2 all together with two buffer sets); -1 = 0,
 //   T   FFT twiddles: 1 per-stage tables copied to LDS (lanes read
@@ -464,7 +465,9 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   // group, phi, twiddles): one memory latency instead of one per phase.
   static_assert(B * L == NT * E, "each thread moves E elements of the tile");
   constexpr int G = Cfg::G;
-  constexpr bool PF = Cfg::PF != 0 && (MODE == ROW_P1 || MODE == ROW_P3);
+  // phi: 1 loaded with the tile, 2 after the tile's barrier, 0 where used
+  constexpr int PFM = (MODE == ROW_P1 || MODE == ROW_P3) ? Cfg::PF : 0;
+  constexpr bool PF = PFM == 1;
   constexpr bool PHI_HINT = std::is_same<CT, double>::value && N >= PHI_HINT_MIN_N;
   // tile -> component (split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2
   // TILES) take Gy/Jy) and first row
@@ -612,6 +615,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
         tile_to_lds(g0, tl);
       }
       lds_barrier();   // the prefetched phi row stays in flight
+      if constexpr (PFM == 2) load_phi(rbase, ph);
       c2r();
     }
 
@@ -625,7 +629,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
       }
       return;
     }
-    if constexpr ((MODE == ROW_P1 || MODE == ROW_P3) && !PF) load_phi(rbase, ph);
+    if constexpr ((MODE == ROW_P1 || MODE == ROW_P3) && PFM == 0) load_phi(rbase, ph);
     pointwise_r2c_store(arr0, j0, ph);
   }
 

@@ -2,7 +2,8 @@
 // P4 column) per grid size and precision, fastest first.
 //
 // Each entry is Sched<E, W, G, PF, X, T> (elements per thread, rows or lines
-// per workgroup, tile-load group size with 0 = all at once, prefetch, FFT
+// per workgroup, tile-load group size with 0 = all at once, prefetch (0 where
+// used, 1 with the first loads, 2 once they have arrived), FFT
 // exchange layout, twiddle source with 1 = per-stage tables, 2 = the half
 // table staged in LDS although it exceeds 16 KiB).  The (E, W, G,
 // PF) choices are the three fastest distinct (E, W) of an exhaustive sweep
@@ -17,7 +18,11 @@
 // the loop's order wins.  The first entry is the default; the runtime tuner
 // (csg_autotune, csg_batch_autotune) times the candidates inside the captured
 // step loop and keeps the fastest combination.  Sizes without an entry use the
-// single rule-based schedule Sched<>.
+// single rule-based schedule Sched<>.  After the twiddle staging became
+// branch-free, the 256 .. 4096 lists were swept again in the captured loop
+// (one pass varied at a time, the others at their defaults, with prefetch
+// mode 2 added) and reordered where that sweep and an alternating A/B of the
+// whole step put another candidate first; those comments give in-loop times.
 #pragma once
 #include "csg_kernels.hpp"
 
@@ -84,14 +89,14 @@ template <> struct StepCands<768, false, 4> { using type = SchedList<Sched<12, 2
 // although they lose alone (isolated 11.2 us for Sched<8, 4, ...>); with
 // the LDS-only exchange barrier, issuing the whole tile at once (G = E)
 // is faster again: in the loop 13.4 vs 13.9 us, step 50.2 vs 51.0 us
-template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 0, 1, 0, 1>, Sched<4, 2, 0, 0, 2, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 4, 0, 0, 2, 1>>; };  // in the loop, after the branch-free twiddle staging: 48.7 / 48.8 / 49.6 / 49.0 / 48.9 us per step (the first against the third, alternating: 47.7 vs 48.4)
+template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 2, 0, 1>, Sched<8, 4, 1, 1, 0, 1>, Sched<8, 4, 0, 1, 0, 1>, Sched<4, 2, 0, 0, 2, 1>>; };  // whole step in the loop, alternating: 48.0 / 49.0 / 48.4 / - / 48.7 us (one at a time: 48.8 / 48.3 / 48.3 / 48.8 / 49.6)
 template <> struct StepCands<1024, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<16, 2, 0, 1, 0, 0>>; };  // 13.1 / 14.1 / 14.8 us
 template <> struct StepCands<1024, true, 3> { using type = SchedList<Sched<8, 8, 1, 0, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 0, 0>, Sched<8, 4, 1, 0, 0, 1, 0>>; };  // 14.6 / 14.9 / 16.6 us; in the loop 14.0 / 15.5 / 19.9, and 17.1 for the last (both components, phi read once)
-template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // 10.4 / 11.1 / 12.4 us
-template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4, 1, 0, 0, 0>, Sched<4, 2, 0, 0, 2, 1>, Sched<4, 2, 1, 1, 2, 1>, Sched<8, 4, 1, 1, 0, 1>, Sched<8, 2, 0, 0, 0, 0>>; };  // 8.6 / - / - / - / 9.4 us; in the loop after the branch-free twiddle staging the first four are within 0.1 us per step of each other (36.8-36.9 us with P3 candidate 1)
+template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 2, 0, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // in the loop 12.4 / 12.6 us (whole step 48.0 / 48.1); alone 10.4 / 11.1 / 12.4 us (the last three)
+template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4, 1, 2, 0, 1>, Sched<8, 4, 1, 0, 0, 0>, Sched<4, 2, 0, 0, 2, 1>, Sched<8, 4, 1, 1, 0, 1>, Sched<8, 2, 0, 0, 0, 0>>; };  // in the loop: 37.1 / 37.4 / 37.6 / 37.5 / 38.3 us per step; with P4 candidate 0 against both old defaults, alternating: 36.1 vs 36.9 us
 template <> struct StepCands<1024, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 1, 0, 0>>; };  // 8.6 / 9.4 / 10.7 us
 template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 0, 0, 2, 1>, Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // in the loop 11.3 / 11.5 / 11.9 / 13.9 us, and 14.3 for the last (both components, phi read once); the first two equal after the branch-free twiddle staging
-template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // 7.7 / - / 8.8 / 10.5 us; in the loop after the branch-free twiddle staging 37.5 / 37.3 us per step
+template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 2, 2, 1>, Sched<4, 1, 0, 0, 2, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // in the loop 9.6 / 9.9 / 10.1 / - / 13.8 us
 template <> struct StepCands<2048, true, 1> { using type = SchedList<Sched<4, 1, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // 49.6 / 51.0 / 56.9 us
 template <> struct StepCands<2048, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 0>, Sched<16, 1, 0, 1, 0, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // 44.8 / 51.4 / 52.9 us
 template <> struct StepCands<2048, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 0>, Sched<16, 4, 1, 0, 0, 0>, Sched<4, 1, 1, 0, 0, 1>>; };  // 60.0 / 67.0 / 72.3 us
@@ -106,8 +111,8 @@ template <> struct StepCands<2048, false, 4> { using type = SchedList<Sched<8, 1
 // event times (A/B on one box)
 template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 0, 2, 2>, Sched<8, 1, 0, 1, 2, 2>, Sched<8, 1, 0, 1, 2, 0>, Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 0, 0, 2>>; };  // in the loop 248.4 / 261.5 / 269.6 / 282.7 / 248.0 us
 template <> struct StepCands<4096, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 2>, Sched<8, 1, 0, 1, 1, 0>, Sched<16, 1, 0, 1, 1, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // in the loop 181.2 / 183.7 / 195.5 / 284.9 us
-template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 1, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>, Sched<8, 2, 1, 0, 0, 2>>; };  // in the loop 325.5 / 332.2 / 349.4 / 379.8 / 339.2 us
-template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 0>>; };  // in the loop 169.0 / 173.1 / 187.8 / 185.1 / 226.6 us
+template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 1, 2, 0, 2>, Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 2, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>>; };  // in the loop 305 / 316 / 302 / 349 / 380 us; with P4 candidate 0 against both old defaults, alternating: step 877 vs 902 us
+template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>>; };  // in the loop 159 / 166 / 170 / 188 / 185 us
 template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<8, 2, 0, 0, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 0, 2, 1>>; };  // 115.8 / 138.6 / 138.8 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
 template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 1>, Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // in the loop 132.3 / 137.3 / 159.0 / 174.3 us
