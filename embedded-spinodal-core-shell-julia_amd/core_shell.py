@@ -134,7 +134,9 @@ class Session:
 
     def schedule_info(self) -> list:
         """Per pass, the compiled candidates as dicts (E = elements per thread,
-        W = rows or lines per workgroup, G = tile-load group, PF = prefetch,
+        W = rows or lines per workgroup, G = tile-load group, PF = when the
+        second input is loaded (0 where used, 1 with the first loads, 2 once
+        they have arrived),
         X = FFT exchange layout, T = 1 for per-stage twiddle tables and 2
         for the half table staged in LDS beyond 16 KiB, S = 1
         when P3 gives each gradient component its own workgroups, 0 when one

@@ -174,7 +174,9 @@ int csg_set_params(csg_ctx* ctx, const csg_params* p);
  *
  * csg_schedule_info: count of candidates of `pass`; `sched` (may be NULL,
  * else 7 * count int32) receives each candidate's [elements per thread, rows
- * or lines per workgroup, tile-load group, prefetch, FFT exchange layout,
+ * or lines per workgroup, tile-load group, prefetch (when the second input
+ * is loaded: 0 where used, 1 with the first loads, 2 once they have arrived),
+ * FFT exchange layout,
  * twiddle source (1 = per-stage tables, 2 = half table in LDS beyond 16 KiB),
  * P3 split (1 = one gradient component
  * per workgroup; 0 for the other passes)]. */
