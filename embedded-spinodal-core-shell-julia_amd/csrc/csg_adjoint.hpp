@@ -159,7 +159,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   double r[1] = {acc};
   const int ops[1] = {RED_SUM};
-  block_reduce<NT, 1>(r, ops, scratch);
+  block_reduce0<NT, 1>(r, ops, scratch);
   if (threadIdx.x == 0) a.acc[blockIdx.x] += r[0] * dt / ((double)N * N);
 }
 
@@ -236,7 +236,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   double rr[1] = {acc};
   const int ops[1] = {RED_SUM};
-  block_reduce<NT, 1>(rr, ops, scratch);
+  block_reduce0<NT, 1>(rr, ops, scratch);
   if (threadIdx.x == 0) a.acc[blockIdx.x] += rr[0] * 2.0 * dt / ((double)N * N);
 }
 
@@ -380,7 +380,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
   }
   double rr[1] = {srg};
   const int ops[1] = {RED_SUM};
-  block_reduce<NT, 1>(rr, ops, scratch);
+  block_reduce0<NT, 1>(rr, ops, scratch);
   if (threadIdx.x == 0) a.acc[blockIdx.x] += rr[0] * dt;
 }
 

@@ -571,4 +571,41 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], const int (&ops)[K]
   __syncthreads();
 }
 
+// Reduce K values across the workgroup into thread 0, in the same order as
+// block_reduce (bit-identical results); the other threads' values are left
+// unspecified.  One barrier (none for a single wavefront) and no barrier at
+// the end: for the partials a pass writes once, when it is done with
+// `scratch` (a caller that reuses the LDS afterwards synchronises first).
+template <int NT, int K>
+__device__ __forceinline__ void block_reduce0(double (&v)[K], const int (&ops)[K], double* scratch) {
+  constexpr int NW = (NT + 63) / 64;
+  constexpr int W = NT < 64 ? NT : 64;
+  static_assert((W & (W - 1)) == 0 && (NT <= 64 || NT % 64 == 0), "whole wavefronts or one power-of-two part");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (ops[q] == RED_SUM) v[q] = wave_reduce<RED_SUM, W>(v[q]);
+    else if (ops[q] == RED_MIN) v[q] = wave_reduce<RED_MIN, W>(v[q]);
+    else v[q] = wave_reduce<RED_MAX, W>(v[q]);
+  }
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) scratch[q * NW + wave] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        double a = scratch[q * NW];
+        for (int w = 1; w < NW; ++w) {
+          const double b = scratch[q * NW + w];
+          a = (ops[q] == RED_SUM) ? a + b : (ops[q] == RED_MIN ? red_combine<RED_MIN>(a, b) : red_combine<RED_MAX>(a, b));
+        }
+        v[q] = a;
+      }
+    }
+  }
+}
+
 }  // namespace csg

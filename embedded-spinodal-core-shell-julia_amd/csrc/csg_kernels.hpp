@@ -656,7 +656,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
 
   if constexpr (MODE == ROW_P1) {
     const int ops[5] = {RED_SUM, RED_SUM, RED_MIN, RED_MAX, RED_SUM};
-    block_reduce<NT, 5>(red, ops, scratch);
+    block_reduce0<NT, 5>(red, ops, scratch);
     if (threadIdx.x == 0) {
       double* rec = a.part + 5 * bx;
 #pragma unroll
@@ -666,7 +666,7 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   if constexpr (MODE == ROW_P3) {
     double r[1] = {red[0]};
     const int ops[1] = {RED_SUM};
-    block_reduce<NT, 1>(r, ops, scratch);
+    block_reduce0<NT, 1>(r, ops, scratch);
     if (threadIdx.x == 0) a.part[bx] = r[0];
   }
 }
