@@ -139,8 +139,10 @@ class Session:
         they have arrived),
         X = FFT exchange layout, T = 1 for per-stage twiddle tables and 2
         for the half table staged in LDS beyond 16 KiB, S = 1
-        when P3 gives each gradient component its own workgroups, 0 when one
-        workgroup transforms both; 0 for the other passes)."""
+        when P3 gives each gradient component its own workgroups, 2 when it
+        does so with the two components of a row tile on neighbouring
+        workgroups, 0 when one workgroup transforms both; 0 for the other
+        passes)."""
         keys = ("E", "W", "G", "PF", "X", "T", "S")
         out = []
         for q in range(4):

@@ -302,7 +302,8 @@ constexpr int row_rows() {
 //       whatever its size; -1 = 0,
 //   S   P3 only: 1 gives each gradient component its own workgroups (phi is
 //       then read once per component), 0 transforms both components of W
-//       rows in one workgroup (phi read once).
+//       rows in one workgroup (phi read once), 2 is 1 with the two
+//       components of the same rows on neighbouring workgroups.
 template <int E_ = 0, int W_ = 0, int G_ = 0, int PF_ = 1, int X_ = -1, int T_ = -1, int S_ = 1>
 struct Sched {
   static constexpr int E = E_, W = W_, G = G_, PF = PF_, X = X_, T = T_, S = S_;
@@ -471,8 +472,16 @@ __device__ __forceinline__ void row_body(const PassArgs& a0, int bx, int q, cons
   constexpr bool PHI_HINT = std::is_same<CT, double>::value && N >= PHI_HINT_MIN_N;
   // tile -> component (split P3: tiles [0, TILES) take Gx/Jx, [TILES, 2
   // TILES) take Gy/Jy) and first row
-  auto arr_of = [&](int tile) { return (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0; };
-  auto row0_of = [&](int tile) { return (tile - arr_of(tile) * Cfg::TILES) * B; };
+  // (S = 2: tiles 2t and 2t+1 take Gx and Gy of the same rows, so the XCD
+  // that runs one also runs the other and the second phi read hits its L2)
+  auto arr_of = [&](int tile) {
+    if constexpr (Cfg::SPLIT && SC::S == 2) return tile & 1;
+    else return (Cfg::SPLIT && tile >= Cfg::TILES) ? 1 : 0;
+  };
+  auto row0_of = [&](int tile) {
+    if constexpr (Cfg::SPLIT && SC::S == 2) return (tile >> 1) * B;
+    else return (tile - arr_of(tile) * Cfg::TILES) * B;
+  };
   auto load_phi = [&](size_t rbase, CT (&ph)[E][2]) {
     const CT* phi = static_cast<const CT*>(a.phi);
     // The NT_PHI branch is compiled only where the hint can be set (FP64

@@ -97,7 +97,7 @@ void add_cand(PassCands& pc) {
     pc.fn[i] = launch_row<T, TIN, T, N, MODE, SC>;
     pc.grid[i] = Cfg::GRID;
     const int r[SCHED_FIELDS] = {Cfg::E, Cfg::B, Cfg::G, Cfg::PF, Cfg::XM, Cfg::ST ? 1 : (SC::T == 2 ? 2 : 0),
-                                 PASS == 3 ? (Cfg::SPLIT ? 1 : 0) : 0};
+                                 PASS == 3 ? (Cfg::SPLIT ? SC::S : 0) : 0};
     for (int q = 0; q < SCHED_FIELDS; ++q) pc.sched[i][q] = r[q];
   } else {
     constexpr int MODE = PASS == 2 ? COL_P2 : COL_P4;

@@ -115,7 +115,7 @@ template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2,
 template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>>; };  // in the loop 159 / 166 / 170 / 188 / 185 us
 template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<8, 2, 0, 0, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 0, 2, 1>>; };  // 115.8 / 138.6 / 138.8 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
-template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 1>, Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // in the loop 132.3 / 137.3 / 159.0 / 174.3 us
+template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 1, 2>, Sched<16, 4, 1, 0, 0, 1>, Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // the first: both components of the same rows on neighbouring workgroups, step 408.5 -> 405.9 us (alternating A/B); then in the loop 132.3 / 137.3 / 159.0 / 174.3 us
 template <> struct StepCands<4096, false, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 1>, Sched<16, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 1>>; };  // 80.0 / 84.8 / 121.9 us
 
 }  // namespace csg

@@ -178,8 +178,9 @@ int csg_set_params(csg_ctx* ctx, const csg_params* p);
  * is loaded: 0 where used, 1 with the first loads, 2 once they have arrived),
  * FFT exchange layout,
  * twiddle source (1 = per-stage tables, 2 = half table in LDS beyond 16 KiB),
- * P3 split (1 = one gradient component
- * per workgroup; 0 for the other passes)]. */
+ * P3 split (1 = one gradient component per workgroup, 2 = the same with the
+ * two components of a row tile on neighbouring workgroups, 0 = both in one;
+ * 0 for the other passes)]. */
 int csg_schedule_info(const csg_ctx* ctx, int32_t pass, int32_t* count, int32_t* sched);
 /* Selected candidate per pass (sel[0..3]). */
 int csg_get_schedule(const csg_ctx* ctx, int32_t* sel);
