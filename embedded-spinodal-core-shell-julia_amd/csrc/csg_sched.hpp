@@ -89,7 +89,7 @@ template <> struct StepCands<768, false, 4> { using type = SchedList<Sched<12, 2
 // although they lose alone (isolated 11.2 us for Sched<8, 4, ...>); with
 // the LDS-only exchange barrier, issuing the whole tile at once (G = E)
 // is faster again: in the loop 13.4 vs 13.9 us, step 50.2 vs 51.0 us
-template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<4, 2, 1, 0, 2, 1>, Sched<8, 4, 1, 2, 0, 1>, Sched<8, 4, 1, 1, 0, 1>, Sched<8, 4, 0, 1, 0, 1>, Sched<4, 2, 0, 0, 2, 1>>; };  // whole step in the loop, alternating: 48.0 / 49.0 / 48.4 / - / 48.7 us (one at a time: 48.8 / 48.3 / 48.3 / 48.8 / 49.6)
+template <> struct StepCands<1024, true, 1> { using type = SchedList<Sched<8, 4, 1, 2, 2, 1>, Sched<8, 4, 1, 2, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 4, 1, 2, 0, 0>, Sched<4, 2, 1, 0, 2, 1>>; };  // after the partial sums were pinned ahead of the forward FFT, in the loop (one pass varied, whole step): 45.75 / 45.81 / 46.12 / 46.23 / 46.81 us
 template <> struct StepCands<1024, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<16, 2, 0, 1, 0, 0>>; };  // 13.1 / 14.1 / 14.8 us
 template <> struct StepCands<1024, true, 3> { using type = SchedList<Sched<8, 8, 1, 0, 0, 1>, Sched<8, 4, 1, 0, 0, 1>, Sched<8, 2, 1, 0, 0, 0>, Sched<8, 4, 1, 0, 0, 1, 0>>; };  // 14.6 / 14.9 / 16.6 us; in the loop 14.0 / 15.5 / 19.9, and 17.1 for the last (both components, phi read once)
 template <> struct StepCands<1024, true, 4> { using type = SchedList<Sched<4, 1, 0, 2, 0, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 1, 1>, Sched<16, 2, 0, 0, 0, 0>>; };  // in the loop 12.4 / 12.6 us (whole step 48.0 / 48.1); alone 10.4 / 11.1 / 12.4 us (the last three)
@@ -109,11 +109,11 @@ template <> struct StepCands<2048, false, 4> { using type = SchedList<Sched<8, 1
 // of read through the caches in every FFT stage, with the (G, PF, X) that
 // then time best; candidates with and without it ranked by their in-loop
 // event times (A/B on one box)
-template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 0, 2, 2>, Sched<8, 1, 0, 1, 2, 2>, Sched<8, 1, 0, 1, 2, 0>, Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 0, 0, 2>>; };  // in the loop 248.4 / 261.5 / 269.6 / 282.7 / 248.0 us
+template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 2, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<4, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>>; };  // after the partial sums were pinned ahead of the forward FFT (two workgroups per CU with one exchange buffer), in the loop: 172.4 / 176.7 / 186.8 / 190.5 / 192.7 us (the earlier default <8, 1, 0, 0, 2, 2>: 231.8)
 template <> struct StepCands<4096, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 2>, Sched<8, 1, 0, 1, 1, 0>, Sched<16, 1, 0, 1, 1, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // in the loop 181.2 / 183.7 / 195.5 / 284.9 us
 template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 1, 2, 0, 2>, Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 2, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>>; };  // in the loop 305 / 316 / 302 / 349 / 380 us; with P4 candidate 0 against both old defaults, alternating: step 877 vs 902 us
 template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>>; };  // in the loop 159 / 166 / 170 / 188 / 185 us
-template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<8, 2, 0, 0, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 0, 2, 1>>; };  // 115.8 / 138.6 / 138.8 us
+template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<16, 2, 0, 0, 0, 0>, Sched<8, 2, 0, 0, 0, 0>, Sched<8, 4, 0, 0, 0, 0>, Sched<8, 2, 0, 0, 2, 1>, Sched<8, 2, 0, 0, 0, 1>>; };  // after the partial sums were pinned ahead of the forward FFT, in the loop: 105.5 / 112.1 / 118.4 / 127.3 / 130.5 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
 template <> struct StepCands<4096, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 1, 2>, Sched<16, 4, 1, 0, 0, 1>, Sched<16, 4, 1, 1, 0, 1>, Sched<8, 2, 1, 1, 0, 0>, Sched<16, 2, 1, 0, 0, 0>>; };  // the first: both components of the same rows on neighbouring workgroups, step 408.5 -> 405.9 us (alternating A/B); then in the loop 132.3 / 137.3 / 159.0 / 174.3 us
 template <> struct StepCands<4096, false, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 1>, Sched<16, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 1>>; };  // 80.0 / 84.8 / 121.9 us
