@@ -10,7 +10,7 @@ namespace csg {
 
 using LaunchFn = void (*)(const PassArgs&, const void* tw, hipStream_t);
 
-constexpr int MAX_CANDS = 5;
+constexpr int MAX_CANDS = 7;
 constexpr int SCHED_FIELDS = 7;   // E, W, G, PF, X, T, S
 
 // Compiled schedules of one step pass.

@@ -167,7 +167,7 @@ int csg_set_params(csg_ctx* ctx, const csg_params* p);
 /* ---- kernel schedules ------------------------------------------------------
  * No counterpart in the reference (FFTW plans its transforms itself).  Each
  * of the four pass kernels of a step (pass 0..3 = P1 row, P2 column, P3 row,
- * P4 column) is compiled in up to 5 candidate schedules, fastest first as
+ * P4 column) is compiled in up to 7 candidate schedules, fastest first as
  * measured on MI355X; candidate 0 is the default.  All candidates compute the
  * same step (they differ in the FFT radix plan, so results agree to rounding,
  * not bit for bit).

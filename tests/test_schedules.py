@@ -56,7 +56,7 @@ def test_every_candidate_matches_oracle_fp64(pkg, n):
     cr, hr = o.run_simulation_fast(c, phi, steps=steps)
     s = pkg.Session(n)
     info = s.schedule_info()
-    assert [len(x) for x in info] and all(1 <= len(x) <= 5 for x in info)
+    assert [len(x) for x in info] and all(1 <= len(x) <= 7 for x in info)
     for q in range(4):
         for i, sc in enumerate(info[q]):
             sel = [0, 0, 0, 0]
