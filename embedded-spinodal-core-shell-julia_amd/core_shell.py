@@ -183,13 +183,18 @@ class Session:
         self._check_shape(c0)
         self._check_shape(phi)
         check(lib().csg_set_fields(self.handle, ptr(c0), ptr(phi)))
-        self._phi_key = _key(phi)
+        # phi is not fingerprinted here (hashing 8 MB at 1024^2 took longer
+        # than the upload): the next ensure_phi uploads its phi again
+        self._phi_key = None
 
-    def ensure_phi(self, phi):
+    def ensure_phi(self, phi, key=None):
+        """Upload phi (as both fields) unless the phi with fingerprint `key`
+        (computed here when None) is the resident one."""
         phi = _fortran(phi)
-        k = _key(phi)
+        k = _key(phi) if key is None else key
         if k != self._phi_key:
             self.set_fields(phi, phi)
+            self._phi_key = k
 
     def _check_shape(self, a):
         if a.shape != (self.n, self.n):
@@ -264,21 +269,24 @@ class Shape:
     g: np.ndarray
     M: np.ndarray
     phi: np.ndarray
+    # fingerprint of phi, taken once by geometry (None: taken when needed)
+    key: tuple = dataclasses.field(default=None, repr=False, compare=False)
 
 
 def geometry(phi, p: Parameters = Parameters(), *, precision: str = "fp64", device: int = 0) -> Shape:
     phi = _fortran(phi)
+    key = _key(phi)
     s = _session(phi.shape[0], 1.0, p, precision, device)
-    s.ensure_phi(phi)
+    s.ensure_phi(phi, key)
     n = phi.shape[0]
     h, g, M = (np.empty((n, n), order="F") for _ in range(3))
     check(lib().csg_geometry(s.handle, ptr(h), ptr(g), ptr(M)))
-    return Shape(h, g, M, phi)
+    return Shape(h, g, M, phi, key)
 
 
 def _ctx(shape: Shape, spacing: float, p: Parameters, precision: str, device: int) -> Session:
     s = _session(shape.phi.shape[0], spacing, p, precision, device)
-    s.ensure_phi(shape.phi)
+    s.ensure_phi(shape.phi, shape.key)
     return s
 
 

@@ -151,6 +151,7 @@ adj_col1(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     v[0][j] = mul_i(sx * V);
     v[1][j] = mul_i(sy * V);
   }
+  pin_partial(acc);
   fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(v, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -227,6 +228,7 @@ adj_col3(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     w[0][j] = (1.0 / N) * r;
     w[1][j] = (dt / N) * lap;
   }
+  pin_partial(acc);
   fft_regs<double, +1, N, 2, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, line, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) {
@@ -353,6 +355,7 @@ adj_row4(AdjArgs a, const cplx<double>* __restrict__ tw_g) {
     }
     w[0][j] = mk(out[0], out[1]);
   }
+  pin_partial(srg);
   fft_regs<double, -1, L, 1, N, WAVE, E, Cfg::XM, Cfg::AS, Cfg::BS>(w, t, row, tw);
 #pragma unroll
   for (int j = 0; j < E; ++j) row[pad<double, L>(t + TPL * j)] = w[0][j];

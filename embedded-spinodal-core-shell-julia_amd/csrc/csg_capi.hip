@@ -54,6 +54,12 @@ struct csg_ctx {
   // non-finite checks during long runs: pinned slots for ctr[1] and events
   long long* flag_pin = nullptr;
   hipEvent_t flag_ev[2] = {nullptr, nullptr};
+  // uploads from pageable host memory: two pinned chunks and the events of
+  // their last transfers (made on first use)
+  void* up_pin[2] = {nullptr, nullptr};
+  hipEvent_t up_ev[2] = {nullptr, nullptr};
+  bool up_busy[2] = {false, false};
+  int up_next = 0;
   double* scratch_row = nullptr;
   double* rate = nullptr; // (N/2+1) x N complex doubles
   bool fields_set = false;
@@ -196,10 +202,46 @@ void enqueue_iteration(csg_ctx* c, double dt, bool advance, int64_t every) {
   else enqueue_finalize(c, c->hist, dt, every);
 }
 
-int upload_real(csg_ctx* c, const double* host) {
-  const size_t nn = static_cast<size_t>(c->n) * c->n;
-  HIPCHK(hipMemcpyAsync(c->real, host, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+// Uploads from pageable host memory go through two pinned chunks of
+// UPLOAD_CHUNK bytes: the host fills one (converting to T on the way) while
+// the copy engine reads the other.  hipMemcpyAsync straight from pageable
+// memory moved the 16 MB of csg_set_fields at 1024^2 in 6 ms, with the GPU
+// idle and its clocks falling meanwhile, so the run that followed started
+// slower.  A chunk is refilled only once the event after its last transfer
+// has completed, also across calls; on return the copies are queued on the
+// context's stream and `src` is no longer read.
+constexpr size_t UPLOAD_CHUNK = size_t(2) << 20;
+
+template <class T>
+int upload(csg_ctx* c, T* dst, const double* src, size_t count) {
+  for (int b = 0; b < 2; ++b) {
+    if (!c->up_pin[b]) HIPCHK(hipHostMalloc(&c->up_pin[b], UPLOAD_CHUNK, hipHostMallocDefault));
+    if (!c->up_ev[b]) HIPCHK(hipEventCreateWithFlags(&c->up_ev[b], hipEventDisableTiming));
+  }
+  constexpr size_t PER = UPLOAD_CHUNK / sizeof(T);
+  for (size_t off = 0; off < count; off += PER) {
+    const int b = c->up_next;
+    c->up_next ^= 1;
+    if (c->up_busy[b]) {
+      HIPCHK(hipEventSynchronize(c->up_ev[b]));
+      c->up_busy[b] = false;
+    }
+    const size_t len = std::min(PER, count - off);
+    T* pin = static_cast<T*>(c->up_pin[b]);
+    if constexpr (std::is_same<T, double>::value) {
+      std::memcpy(pin, src + off, len * sizeof(double));
+    } else {
+      for (size_t i = 0; i < len; ++i) pin[i] = static_cast<T>(src[off + i]);
+    }
+    HIPCHK(hipMemcpyAsync(dst + off, pin, len * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->up_ev[b], c->stream));
+    c->up_busy[b] = true;
+  }
   return CSG_OK;
+}
+
+int upload_real(csg_ctx* c, const double* host) {
+  return upload(c, c->real, host, static_cast<size_t>(c->n) * c->n);
 }
 
 int set_state_from_real(csg_ctx* c, void* S) {
@@ -407,6 +449,10 @@ int csg_destroy(csg_ctx* c) {
   if (c->hstage) (void)hipHostFree(c->hstage);
   for (hipEvent_t ev : c->flag_ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (int b = 0; b < 2; ++b) {
+    if (c->up_pin[b]) (void)hipHostFree(c->up_pin[b]);
+    if (c->up_ev[b]) (void)hipEventDestroy(c->up_ev[b]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CSG_OK;
@@ -434,12 +480,9 @@ int csg_set_fields(csg_ctx* c, const double* c0, const double* phi) {
   HIPCHK(hipSetDevice(c->device));
   const size_t nn = static_cast<size_t>(c->n) * c->n;
   if (c->precision == CSG_FP64) {
-    HIPCHK(hipMemcpyAsync(c->phi, phi, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (int rc = upload(c, static_cast<double*>(c->phi), phi, nn)) return rc;
   } else {
-    std::vector<float> tmp(nn);
-    for (size_t i = 0; i < nn; ++i) tmp[i] = static_cast<float>(phi[i]);
-    HIPCHK(hipMemcpyAsync(c->phi, tmp.data(), nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (int rc = upload(c, static_cast<float*>(c->phi), phi, nn)) return rc;
   }
   if (int rc = upload_real(c, c0)) return rc;
   if (int rc = set_state_from_real(c, c->S)) return rc;

@@ -141,10 +141,11 @@ template <class T> __device__ __forceinline__ bool finite(T v) { return isfinite
 // and the stores, keeping every site's intermediate values live across the
 // transform (4096^2 FP64 P1: 256 VGPRs plus 94 AGPRs, one wave per SIMD).
 // The empty asm has side effects, so it stays ahead of the FFT's barriers.
+__device__ __forceinline__ void pin_partial(double& r) { asm volatile("" : "+v"(r)); }
 template <int K>
 __device__ __forceinline__ void pin_partials(double (&r)[K]) {
 #pragma unroll
-  for (int q = 0; q < K; ++q) asm volatile("" : "+v"(r[q]));
+  for (int q = 0; q < K; ++q) pin_partial(r[q]);
 }
 
 // Line blocking of the line layout (a build option, off by default).  The L

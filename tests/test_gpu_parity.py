@@ -81,6 +81,28 @@ def test_chemical_potential_matches_oracle(pkg, n, spacing):
     assert np.max(np.abs(mu - mr)) < 1e-12 * max(1.0, np.max(np.abs(mr)))
 
 
+def test_function_api_alternating_particles(pkg):
+    """Function-API calls that alternate between two particles of one size
+    (one cached device context) each use their own phi, also after a
+    run_simulation on a third particle replaced the resident fields."""
+    say("function API, alternating particles at 64^2")
+    n = 64
+    c, phi1 = _fields(pkg, n, radius=16.0)
+    _, phi2 = _fields(pkg, n, radius=22.0, width=2.0)
+    _, phi3 = _fields(pkg, n, radius=10.0)
+    g = pkg.make_grid(n, 1.0)
+    s1, s2 = pkg.geometry(phi1), pkg.geometry(phi2)
+    for shape, phi in ((s1, phi1), (s2, phi2), (s1, phi1), (None, phi3), (s2, phi2), (s1, phi1)):
+        if shape is None:
+            pkg.run_simulation(c, phi, dt=0.01, steps=3)
+            continue
+        mu = pkg.chemical_potential(c, shape, g)
+        mr = o.chemical_potential(c, o.geometry(phi, P), o.make_grid(n, 1.0), P)
+        assert np.max(np.abs(mu - mr)) < 1e-12 * max(1.0, np.max(np.abs(mr)))
+        assert abs(pkg.energy(c, shape, g) - o.energy(c, o.geometry(phi, P), o.make_grid(n, 1.0), P)) \
+            < 1e-12 * abs(o.energy(c, o.geometry(phi, P), o.make_grid(n, 1.0), P))
+
+
 @pytest.mark.parametrize("n,spacing", [(16, 1.0), (64, 1.0), (64, 2.0), (256, 1.0), (1024, 1.0)])
 def test_transport_rate_matches_oracle(pkg, n, spacing):
     say(f"transport_rate n={n} spacing={spacing}")
