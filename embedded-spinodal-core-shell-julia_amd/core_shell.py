@@ -274,7 +274,11 @@ class Shape:
 
 
 def geometry(phi, p: Parameters = Parameters(), *, precision: str = "fp64", device: int = 0) -> Shape:
-    phi = _fortran(phi)
+    # a read-only snapshot, like the reference's (h, g, M) computed here: a
+    # later change to the caller's array does not reach the device through
+    # this Shape
+    phi = np.array(phi, dtype=np.float64, order="F", copy=True)
+    phi.setflags(write=False)
     key = _key(phi)
     s = _session(phi.shape[0], 1.0, p, precision, device)
     s.ensure_phi(phi, key)

@@ -84,14 +84,17 @@ def test_chemical_potential_matches_oracle(pkg, n, spacing):
 def test_function_api_alternating_particles(pkg):
     """Function-API calls that alternate between two particles of one size
     (one cached device context) each use their own phi, also after a
-    run_simulation on a third particle replaced the resident fields."""
+    run_simulation on a third particle replaced the resident fields, and a
+    Shape keeps the phi it was made from when the caller's array changes."""
     say("function API, alternating particles at 64^2")
     n = 64
     c, phi1 = _fields(pkg, n, radius=16.0)
     _, phi2 = _fields(pkg, n, radius=22.0, width=2.0)
     _, phi3 = _fields(pkg, n, radius=10.0)
     g = pkg.make_grid(n, 1.0)
-    s1, s2 = pkg.geometry(phi1), pkg.geometry(phi2)
+    mine = phi1.copy(order="F")
+    s1, s2 = pkg.geometry(mine), pkg.geometry(phi2)
+    mine[:] = 0.0
     for shape, phi in ((s1, phi1), (s2, phi2), (s1, phi1), (None, phi3), (s2, phi2), (s1, phi1)):
         if shape is None:
             pkg.run_simulation(c, phi, dt=0.01, steps=3)
