@@ -43,7 +43,7 @@ mutable struct Context
     ptr::Ptr{Cvoid}
     n::Int
     p::Parameters
-    phi_hash::UInt
+    phi_key::Union{Nothing,UInt}   # fingerprint of the resident phi (nothing: unknown)
     generation::Int
     function Context(n::Integer, spacing::Real, p::Parameters, precision::Symbol, device::Integer)
         out = Ref{Ptr{Cvoid}}(C_NULL)
@@ -53,7 +53,7 @@ mutable struct Context
         check(ccall((:csg_create, libcsg), Cint,
                     (Ref{Ptr{Cvoid}}, Int32, Float64, Ref{Parameters}, Int32, Int32),
                     out, n, spacing, pref, prec, device))
-        ctx = new(out[], Int(n), p, UInt(0), 0)
+        ctx = new(out[], Int(n), p, nothing, 0)
         finalizer(destroy!, ctx)
         return ctx
     end
@@ -81,7 +81,7 @@ end
 function set_fields!(ctx::Context, c0::Matrix{Float64}, phi::Matrix{Float64})
     GC.@preserve c0 phi check(ccall((:csg_set_fields, libcsg), Cint,
                                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.ptr, c0, phi))
-    ctx.phi_hash = hash(phi)
+    ctx.phi_key = nothing   # not fingerprinted here; the next `context` call uploads its phi
     ctx.generation += 1
     return ctx
 end
@@ -143,28 +143,41 @@ function cached_context(n, spacing, p, precision, device; cache=contexts, order=
     return ctx
 end
 
-function context(phi, spacing, p, precision, device)
+# Fingerprint of every element of phi.  Base.hash of an array of 8192 or more
+# elements hashes only about log2(length) of them, mostly near its end, so two
+# particles of one size that differ only near the centre could hash alike and
+# the device would keep the wrong phi.
+phi_fingerprint(phi) = foldl((h, x) -> hash(x, h), phi; init=hash(size(phi)))
+
+# The fingerprint a shape carries (taken once by `geometry`), else computed.
+shape_key(shape) = hasproperty(shape, :key) ? shape.key : phi_fingerprint(shape.phi)
+
+function context(phi, spacing, p, precision, device; key=phi_fingerprint(phi))
     ctx = cached_context(size(phi, 1), spacing, p, precision, device)
-    if ctx.phi_hash != hash(phi)
+    if ctx.phi_key != key
         m = Matrix{Float64}(phi)
         set_fields!(ctx, m, m)
+        ctx.phi_key = key
     end
     return ctx
 end
 
-# geometry (core_shell.jl:39-43); the shape keeps phi for the device calls.
+# geometry (core_shell.jl:39-43); the shape keeps a snapshot of phi, and its
+# fingerprint, for the device calls.
 function geometry(phi, p; precision=:fp64, device=0)
-    ctx = context(phi, 1.0, p, precision, device)
+    snap = Matrix{Float64}(phi)
+    key = phi_fingerprint(snap)
+    ctx = context(snap, 1.0, p, precision, device; key)
     n = size(phi, 1)
     h, g, M = (Matrix{Float64}(undef, n, n) for _ in 1:3)
     GC.@preserve h g M check(ccall((:csg_geometry, libcsg), Cint,
                                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), ctx.ptr, h, g, M))
-    return (; h, g, M, phi=Matrix{Float64}(phi))
+    return (; h, g, M, phi=snap, key)
 end
 
 # entry points of the form (ctx, const double* c, double* out)
 function pointwise(sym, c, shape, p, spacing, precision, device)
-    ctx = context(shape.phi, spacing, p, precision, device)
+    ctx = context(shape.phi, spacing, p, precision, device; key=shape_key(shape))
     fptr = Libdl.dlsym(Libdl.dlopen(libcsg), sym)
     cc = Matrix{Float64}(c)
     out = similar(cc)
@@ -181,7 +194,7 @@ chemical_potential(c, shape, grid, p; precision=:fp64, device=0) =
 
 # transport_rate (core_shell.jl:62-70): full n x n FFTW-convention spectrum.
 function transport_rate(c, shape, grid, p; precision=:fp64, device=0)
-    ctx = context(shape.phi, grid.spacing, p, precision, device)
+    ctx = context(shape.phi, grid.spacing, p, precision, device; key=shape_key(shape))
     cc = Matrix{Float64}(c)
     re, im = similar(cc), similar(cc)
     d = Ref(0.0)
@@ -193,7 +206,7 @@ end
 
 # energy (core_shell.jl:78-82)
 function energy(c, shape, grid, p; precision=:fp64, device=0)
-    ctx = context(shape.phi, grid.spacing, p, precision, device)
+    ctx = context(shape.phi, grid.spacing, p, precision, device; key=shape_key(shape))
     cc = Matrix{Float64}(c)
     F = Ref(0.0)
     GC.@preserve cc check(ccall((:csg_energy, libcsg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}),
