@@ -42,6 +42,25 @@ def test_state_roundtrip(pkg, n):
     assert np.max(np.abs(s.field() - c)) < 1e-15
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_upload_roundtrip_2048(pkg, precision):
+    """Uploads of 32 MB fields (16 pinned chunks each, the chunks reused across
+    the phi and c uploads and across calls): the state round-trips, and phi
+    arrives whole (h(phi) over every site, FP32 in the mixed form)."""
+    say(f"upload roundtrip 2048^2 {precision}")
+    n = 2048
+    rng = np.random.default_rng(11)
+    s = pkg.Session(n, precision=precision)
+    tol_h = 1e-14 if precision == "fp64" else 1e-6
+    for k in range(2):
+        c = np.asfortranarray(0.5 + 0.4 * rng.standard_normal((n, n)))
+        phi = np.asfortranarray(rng.random((n, n)))
+        s.set_fields(c, phi)
+        assert np.max(np.abs(s.field() - c)) < 1e-14
+        shape = pkg.geometry(phi, precision=precision)
+        assert np.max(np.abs(shape.h - o.geometry(phi, P)["h"])) < tol_h
+
+
 @pytest.mark.parametrize("n", SIZES)
 def test_energy_matches_oracle(pkg, n):
     say(f"energy n={n}")
