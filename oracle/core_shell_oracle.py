@@ -133,28 +133,12 @@ def bulk_derivative(c, shape, p: Parameters):
 
 
 # 2D transforms (FFTW's fft/ifft convention: unnormalised forward, 1/N^2
-# inverse).  Single-threaded numpy pocketfft by default; set_fft_workers(k)
-# runs scipy's pocketfft over k threads (same transform, for a multi-core CPU
-# timing).
-_FFT_WORKERS = 0
-
-
-def set_fft_workers(k: int) -> None:
-    global _FFT_WORKERS
-    _FFT_WORKERS = int(k)
-
-
+# inverse), numpy pocketfft on one thread.
 def fft(x):
-    if _FFT_WORKERS > 1:
-        import scipy.fft
-        return scipy.fft.fft2(x, workers=_FFT_WORKERS)
     return np.fft.fft2(x)
 
 
 def ifft(x):
-    if _FFT_WORKERS > 1:
-        import scipy.fft
-        return scipy.fft.ifft2(x, workers=_FFT_WORKERS)
     return np.fft.ifft2(x)
 
 
