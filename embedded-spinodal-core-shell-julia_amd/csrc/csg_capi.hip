@@ -212,12 +212,17 @@ void enqueue_iteration(csg_ctx* c, double dt, bool advance, int64_t every) {
 // context's stream and `src` is no longer read.
 constexpr size_t UPLOAD_CHUNK = size_t(2) << 20;
 
-template <class T>
-int upload(csg_ctx* c, T* dst, const double* src, size_t count) {
+int ensure_staging(csg_ctx* c) {
   for (int b = 0; b < 2; ++b) {
     if (!c->up_pin[b]) HIPCHK(hipHostMalloc(&c->up_pin[b], UPLOAD_CHUNK, hipHostMallocDefault));
     if (!c->up_ev[b]) HIPCHK(hipEventCreateWithFlags(&c->up_ev[b], hipEventDisableTiming));
   }
+  return CSG_OK;
+}
+
+template <class T>
+int upload(csg_ctx* c, T* dst, const double* src, size_t count) {
+  if (int rc = ensure_staging(c)) return rc;
   constexpr size_t PER = UPLOAD_CHUNK / sizeof(T);
   for (size_t off = 0; off < count; off += PER) {
     const int b = c->up_next;
@@ -242,6 +247,35 @@ int upload(csg_ctx* c, T* dst, const double* src, size_t count) {
 
 int upload_real(csg_ctx* c, const double* host) {
   return upload(c, c->real, host, static_cast<size_t>(c->n) * c->n);
+}
+
+// Downloads to pageable host memory through the same pinned chunks, the
+// copy engine filling one while the host empties the other (stream-ordered
+// after the work queued before).  Returns once `dst` holds all `count` values.
+int download(csg_ctx* c, double* dst, const double* src, size_t count) {
+  if (int rc = ensure_staging(c)) return rc;
+  for (int b = 0; b < 2; ++b)
+    if (c->up_busy[b]) {
+      HIPCHK(hipEventSynchronize(c->up_ev[b]));
+      c->up_busy[b] = false;
+    }
+  constexpr size_t PER = UPLOAD_CHUNK / sizeof(double);
+  const size_t chunks = (count + PER - 1) / PER;
+  for (size_t k = 0; k <= chunks; ++k) {
+    if (k < chunks) {
+      const int b = static_cast<int>(k & 1);
+      const size_t len = std::min(PER, count - k * PER);
+      HIPCHK(hipMemcpyAsync(c->up_pin[b], src + k * PER, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipEventRecord(c->up_ev[b], c->stream));
+    }
+    if (k >= 1) {
+      const int b = static_cast<int>((k - 1) & 1);
+      const size_t len = std::min(PER, count - (k - 1) * PER);
+      HIPCHK(hipEventSynchronize(c->up_ev[b]));
+      std::memcpy(dst + (k - 1) * PER, c->up_pin[b], len * sizeof(double));
+    }
+  }
+  return CSG_OK;
 }
 
 int set_state_from_real(csg_ctx* c, void* S) {
@@ -585,9 +619,7 @@ int csg_get_field(csg_ctx* c, double* out) {
   a.out0 = c->real;
   c->ops.inv_state(a, c->tw64, c->stream);
   HIPCHK(hipGetLastError());
-  const size_t nn = static_cast<size_t>(c->n) * c->n;
-  HIPCHK(hipMemcpyAsync(out, c->real, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (int rc = download(c, out, c->real, static_cast<size_t>(c->n) * c->n)) return rc;
   return CSG_OK;
 }
 
@@ -650,10 +682,7 @@ int csg_chemical_potential(csg_ctx* c, const double* cin, double* mu) {
   a.out0 = c->real;
   c->ops.inv_bulk(a, c->tw, c->stream);
   HIPCHK(hipGetLastError());
-  const size_t nn = static_cast<size_t>(c->n) * c->n;
-  HIPCHK(hipMemcpyAsync(mu, c->real, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return CSG_OK;
+  return download(c, mu, c->real, static_cast<size_t>(c->n) * c->n);
 }
 
 int csg_transport_rate(csg_ctx* c, const double* cin, double* rate_re, double* rate_im,
@@ -1203,7 +1232,7 @@ int csg_adjoint_backward(csg_ctx* c, const double* dL_dcT, double* dL_dc0, doubl
     pa.in0 = c->G;
     pa.out0 = c->real;
     c->ops.inv_state(pa, c->tw64, c->stream);
-    HIPCHK(hipMemcpyAsync(dL_dc0, c->real, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (int rc = download(c, dL_dc0, c->real, bytes / sizeof(double))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   if (dL_dparams) {
