@@ -238,9 +238,9 @@ int upload(csg_ctx* c, T* dst, const double* src, size_t count) {
     } else {
       for (size_t i = 0; i < len; ++i) pin[i] = static_cast<T>(src[off + i]);
     }
+    c->up_busy[b] = true;   // until its event completes, also if a call below fails
     HIPCHK(hipMemcpyAsync(dst + off, pin, len * sizeof(T), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->up_ev[b], c->stream));
-    c->up_busy[b] = true;
   }
   return CSG_OK;
 }
@@ -265,6 +265,7 @@ int download(csg_ctx* c, double* dst, const double* src, size_t count) {
     if (k < chunks) {
       const int b = static_cast<int>(k & 1);
       const size_t len = std::min(PER, count - k * PER);
+      c->up_busy[b] = true;   // until its event completes, also if a call below fails
       HIPCHK(hipMemcpyAsync(c->up_pin[b], src + k * PER, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipEventRecord(c->up_ev[b], c->stream));
     }
@@ -272,6 +273,7 @@ int download(csg_ctx* c, double* dst, const double* src, size_t count) {
       const int b = static_cast<int>((k - 1) & 1);
       const size_t len = std::min(PER, count - (k - 1) * PER);
       HIPCHK(hipEventSynchronize(c->up_ev[b]));
+      c->up_busy[b] = false;
       std::memcpy(dst + (k - 1) * PER, c->up_pin[b], len * sizeof(double));
     }
   }
