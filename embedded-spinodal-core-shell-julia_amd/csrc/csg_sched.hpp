@@ -111,7 +111,7 @@ template <> struct StepCands<2048, false, 4> { using type = SchedList<Sched<8, 1
 // event times (A/B on one box)
 template <> struct StepCands<4096, true, 1> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 2, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<4, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>>; };  // after the partial sums were pinned ahead of the forward FFT (two workgroups per CU with one exchange buffer), in the loop: 172.4 / 176.7 / 186.8 / 190.5 / 192.7 us (the earlier default <8, 1, 0, 0, 2, 2>: 231.8)
 template <> struct StepCands<4096, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 2>, Sched<8, 1, 0, 1, 1, 0>, Sched<16, 1, 0, 1, 1, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // in the loop 181.2 / 183.7 / 195.5 / 284.9 us
-template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<8, 2, 1, 2, 0, 2>, Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 2, 0, 2>, Sched<8, 2, 0, 1, 2, 0>, Sched<4, 2, 1, 1, 2, 0>>; };  // in the loop 305 / 316 / 302 / 349 / 380 us; with P4 candidate 0 against both old defaults, alternating: step 877 vs 902 us
+template <> struct StepCands<4096, true, 3> { using type = SchedList<Sched<16, 2, 0, 2, 0, 2>, Sched<16, 2, 1, 2, 0, 2>, Sched<8, 2, 1, 2, 0, 2>, Sched<8, 2, 1, 1, 0, 2>, Sched<8, 2, 0, 2, 0, 2>, Sched<8, 2, 0, 1, 2, 0>>; };  // E = 16 tried after the partial sums were pinned ahead of the forward FFT: in the loop 281.0 / 282.4 us against 295.8 us for the earlier default (step 797.7 -> 786.1 us); then 305 / 316 / 302 / 349 us as measured before
 template <> struct StepCands<4096, true, 4> { using type = SchedList<Sched<8, 1, 0, 2, 0, 2>, Sched<8, 1, 0, 0, 0, 2>, Sched<8, 1, 0, 1, 0, 2>, Sched<16, 1, 0, 1, 1, 0>, Sched<8, 1, 0, 0, 0, 0>>; };  // in the loop 159 / 166 / 170 / 188 / 185 us
 template <> struct StepCands<4096, false, 1> { using type = SchedList<Sched<16, 2, 0, 0, 0, 0>, Sched<8, 2, 0, 0, 0, 0>, Sched<8, 4, 0, 0, 0, 0>, Sched<8, 2, 0, 0, 2, 1>, Sched<8, 2, 0, 0, 0, 1>>; };  // after the partial sums were pinned ahead of the forward FFT, in the loop: 105.5 / 112.1 / 118.4 / 127.3 / 130.5 us
 template <> struct StepCands<4096, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 1>, Sched<8, 1, 0, 1, 1, 1>, Sched<4, 1, 0, 1, 1, 1>>; };  // 96.1 / 125.8 / 148.8 us
