@@ -97,13 +97,13 @@ template <> struct StepCands<1024, false, 1> { using type = SchedList<Sched<8, 4
 template <> struct StepCands<1024, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 1>, Sched<8, 1, 0, 1, 2, 0>, Sched<16, 2, 0, 1, 0, 0>>; };  // 8.6 / 9.4 / 10.7 us
 template <> struct StepCands<1024, false, 3> { using type = SchedList<Sched<8, 8, 0, 0, 2, 1>, Sched<8, 8, 1, 0, 2, 1>, Sched<8, 4, 1, 0, 2, 1>, Sched<16, 8, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 1, 0>>; };  // in the loop 11.3 / 11.5 / 11.9 / 13.9 us, and 14.3 for the last (both components, phi read once); the first two equal after the branch-free twiddle staging
 template <> struct StepCands<1024, false, 4> { using type = SchedList<Sched<4, 1, 0, 2, 2, 1>, Sched<4, 1, 0, 0, 2, 1>, Sched<4, 1, 0, 0, 0, 1>, Sched<8, 1, 0, 0, 2, 0>, Sched<16, 2, 0, 0, 0, 0>>; };  // in the loop 9.6 / 9.9 / 10.1 / - / 13.8 us
-template <> struct StepCands<2048, true, 1> { using type = SchedList<Sched<4, 1, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // 49.6 / 51.0 / 56.9 us
+template <> struct StepCands<2048, true, 1> { using type = SchedList<Sched<8, 2, 0, 0, 0, 0>, Sched<4, 1, 0, 2, 0, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<8, 1, 0, 2, 0, 0>, Sched<4, 1, 1, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 1>, Sched<8, 2, 1, 0, 2, 0>>; };  // re-swept in the captured loop after the partial sums were pinned ahead of the forward FFT: 36.8 / 43.2 / 44.0 / 45.5 / 46.9 / 50.1 / 52.9 us (step 174.6 -> 163.6 us)
 template <> struct StepCands<2048, true, 2> { using type = SchedList<Sched<8, 1, 0, 1, 0, 0>, Sched<16, 1, 0, 1, 0, 0>, Sched<4, 1, 0, 1, 0, 0>>; };  // 44.8 / 51.4 / 52.9 us
-template <> struct StepCands<2048, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 0>, Sched<16, 4, 1, 0, 0, 0>, Sched<4, 1, 1, 0, 0, 1>>; };  // 60.0 / 67.0 / 72.3 us
+template <> struct StepCands<2048, true, 3> { using type = SchedList<Sched<8, 2, 1, 1, 0, 0>, Sched<16, 4, 1, 0, 0, 0>, Sched<4, 1, 1, 0, 0, 1>>; };  // 60.0 / 67.0 / 72.3 us; in the loop after the pinning 53.2 us for the first, and the 4096^2 winners (E = 16 or 8, two rows, G = 0 or 1, PF = 2) no faster (52.8 - 61.7 us)
 template <> struct StepCands<2048, true, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 0, 0>, Sched<16, 1, 0, 0, 0, 0>>; };  // 38.4 / 44.6 / 45.7 us
-template <> struct StepCands<2048, false, 1> { using type = SchedList<Sched<8, 2, 1, 0, 2, 1>, Sched<4, 1, 0, 1, 2, 1>, Sched<8, 1, 0, 0, 2, 0>>; };  // 27.6 / 28.7 / 30.9 us
+template <> struct StepCands<2048, false, 1> { using type = SchedList<Sched<8, 4, 0, 0, 0, 0>, Sched<8, 2, 0, 0, 0, 0>, Sched<16, 2, 0, 0, 0, 0>, Sched<4, 1, 0, 1, 2, 1>, Sched<8, 2, 1, 0, 2, 1>, Sched<8, 1, 0, 0, 2, 0>>; };  // re-swept in the captured loop after the partial sums were pinned ahead of the forward FFT: 26.3 / 27.2 / 28.0 / 30.5 / 32.3 / 33.9 us (step 109.0 -> 103.5 us)
 template <> struct StepCands<2048, false, 2> { using type = SchedList<Sched<16, 1, 0, 1, 0, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<4, 1, 0, 1, 0, 1>>; };  // 24.6 / 25.0 / 29.4 us
-template <> struct StepCands<2048, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 0>, Sched<16, 8, 1, 0, 0, 0>, Sched<8, 2, 1, 0, 0, 0>>; };  // 30.8 / 32.5 / 34.0 us
+template <> struct StepCands<2048, false, 3> { using type = SchedList<Sched<16, 4, 1, 0, 0, 0>, Sched<16, 8, 1, 0, 0, 0>, Sched<8, 2, 1, 0, 0, 0>>; };  // 30.8 / 32.5 / 34.0 us; in the loop after the pinning 32.1 us for the first, the 4096^2 split variant Sched<16, 4, 1, 0, 0, 1, 2> 34.4 us
 template <> struct StepCands<2048, false, 4> { using type = SchedList<Sched<8, 1, 0, 0, 0, 0>, Sched<16, 1, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 1, 1>>; };  // 23.1 / 24.5 / 27.5 us
 // 4096^2 FP64: the 32 KiB half table of W_4096 staged in LDS (T = 2) instead
 // of read through the caches in every FFT stage, with the (G, PF, X) that

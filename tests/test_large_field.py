@@ -40,6 +40,28 @@ def test_large_field_one_step_matches_oracle(pkg, janus):
     assert np.max(np.abs(h - hr) / np.maximum(1.0, np.abs(hr))) < 1e-12
 
 
+@pytest.mark.parametrize("precision, tol", [("fp64", 1e-12), ("mixed", 5e-6)])
+def test_2048_default_schedules_match_oracle(pkg, precision, tol):
+    """2048^2 (R = 0.3 N, w = 2, default parameters) with the default schedule
+    of every pass: 3 steps against the oracle, FP64 to 1e-12 and the mixed form
+    to its FP32 tolerance.  The other 2048^2 candidates are held to this
+    default by the schedule tests."""
+    n, steps = 2048, 3
+    say(f"2048^2 {precision} {steps} steps vs oracle")
+    c, phi = pkg.initial_fields(n, radius=0.3 * n, width=2.0)
+    c, phi = np.asarray(c), np.asarray(phi)
+    s = pkg.Session(n, 1.0, pkg.Parameters(), precision)
+    assert s.schedule == (0, 0, 0, 0)
+    s.set_fields(c, phi)
+    h = s.run(0.01, steps)
+    c1 = s.field()
+    s.close()
+    cr, hr = o.run_simulation_fast(c, phi, steps=steps)
+    assert np.max(np.abs(c1 - cr)) < tol
+    assert np.max(np.abs(h[:, 1] - hr[:, 1])) < (1e-13 if precision == "fp64" else 1e-12)   # FP64 mean in both forms
+    assert np.max(np.abs(h[:, 2] - hr[:, 2]) / np.abs(hr[:, 2])) < (1e-12 if precision == "fp64" else 1e-6)
+
+
 def test_large_field_invariants_and_mixed_error(pkg, janus):
     p, _, c, phi = janus
     steps = 200
