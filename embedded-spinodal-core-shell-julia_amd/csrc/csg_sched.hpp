@@ -66,7 +66,7 @@ template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 
 template <> struct StepCands<384, true, 4> { using type = SchedList<Sched<6, 1, 0, 1, 2, 0>, Sched<6, 2, 0, 0, 2, 0>, Sched<6, 4, 0, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>>; };  // 7.1 / 7.7 / 9.6 / 10.4 us
 template <> struct StepCands<384, false, 1> { using type = SchedList<Sched<6, 2, 0, 0, 2, 0>, Sched<6, 4, 1, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<6, 8, 1, 0, 2, 1>, Sched<12, 8, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 0, 0>>; };  // 7.1 / 7.2 / 7.9; 64 instances per launch: 46.6 us; appended after the partial sums were pinned ahead of the forward FFT: picked by csg_batch_autotune on the 256-particle ensemble (step 510 -> 497 us mixed, 811 -> 790 us FP64 with all appended entries)
 template <> struct StepCands<384, false, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 0, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>>; };  // 6.6 / 7.2 / 8.7 / 11.4 us
-template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0, 0, 0>, Sched<6, 2, 1, 1, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>>; };  // 6.0 / 6.2 / 6.4 / 6.9 us
+template <> struct StepCands<384, false, 3> { using type = SchedList<Sched<6, 4, 1, 0, 0, 0>, Sched<6, 2, 1, 1, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 0, 0, 0, 0>, Sched<12, 8, 0, 0, 0, 0>>; };  // 6.0 / 6.2 / 6.4 / 6.9 us; appended after the partial sums were pinned ahead of the forward FFT: the one csg_batch_autotune picked on the 256-particle ensemble (mixed step 493-498 -> 489-492 us, alternating A/B)
 template <> struct StepCands<384, false, 4> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 0, 0, 0>, Sched<6, 4, 0, 0, 1, 0>, Sched<12, 2, 0, 0, 0, 0>>; };  // 6.1 / 6.6 / 7.6 / 8.6 us
 template <> struct StepCands<512, true, 1> { using type = SchedList<Sched<4, 1, 0, 0, 2, 1>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 1, 1, 0, 2, 0>, Sched<4, 4, 0, 1, 2, 1>, Sched<4, 4, 0, 1, 0, 1>, Sched<8, 8, 1, 0, 0, 0>>; };  // in the loop after the branch-free twiddle staging: 27.3 / 28.5 us per step for the first two; earlier 6.2 / 6.4 / 6.8 us alone; 64 instances per launch: 128.6 us; appended after the partial sums were pinned ahead of the forward FFT: picked by csg_batch_autotune on the 256-particle ensemble (step 510 -> 497 us mixed, 811 -> 790 us FP64 with all appended entries)
 template <> struct StepCands<512, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 2, 1>, Sched<8, 4, 0, 1, 1, 0>>; };  // 6.4 / 6.6 / 6.8 / 8.2; 64 instances per launch: 108.2 us
@@ -74,7 +74,7 @@ template <> struct StepCands<512, true, 3> { using type = SchedList<Sched<4, 4, 
 template <> struct StepCands<512, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 0, 0, 0>, Sched<8, 2, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 1, 1>, Sched<8, 2, 0, 1, 0, 0>>; };  // 5.7 / 6.1 / 6.2 / 7.2; 64 instances per launch: 94.8 us
 template <> struct StepCands<512, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 1, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 2, 0>, Sched<8, 8, 1, 2, 2, 0>, Sched<8, 4, 1, 2, 0, 1>>; };  // 5.5 / 5.5 / 5.9 / 6.7; 64 instances per launch: 66.2 us; appended after the partial sums were pinned ahead of the forward FFT: picked by csg_batch_autotune on the 256-particle ensemble (step 510 -> 497 us mixed, 811 -> 790 us FP64 with all appended entries)
 template <> struct StepCands<512, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 1, 0>, Sched<8, 1, 0, 1, 1, 0>>; };  // 5.3 / 5.5 / 5.7; 64 instances per launch: 59.2 us
-template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 0>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>, Sched<8, 16, 0, 1, 0, 1>, Sched<8, 16, 0, 1, 0, 0>>; };  // 4.9 / 5.2 / 5.4 / 7.2; 64 instances per launch: 77.3 us
+template <> struct StepCands<512, false, 3> { using type = SchedList<Sched<4, 4, 1, 0, 2, 0>, Sched<4, 2, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>, Sched<8, 16, 0, 1, 0, 1>, Sched<8, 16, 0, 1, 0, 0>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.9 / 5.2 / 5.4 / 7.2; 64 instances per launch: 77.3 us; appended after the partial sums were pinned ahead of the forward FFT: the one csg_batch_autotune picked on the 256-particle ensemble (mixed step 493-498 -> 489-492 us, alternating A/B)
 template <> struct StepCands<512, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<8, 1, 0, 1, 0, 0>, Sched<8, 2, 0, 0, 0, 0>>; };  // 4.8 / 5.1 / 5.5 us
 template <> struct StepCands<768, true, 1> { using type = SchedList<Sched<6, 4, 1, 0, 2, 1>, Sched<6, 1, 1, 0, 2, 0>, Sched<6, 2, 1, 0, 2, 0>>; };  // 11.3 / 11.8 / 12.1 us
 template <> struct StepCands<768, true, 2> { using type = SchedList<Sched<12, 2, 0, 1, 0, 0>, Sched<12, 1, 0, 1, 0, 0>, Sched<6, 1, 0, 1, 2, 0>>; };  // 12.9 / 13.4 / 14.6 us
