@@ -55,11 +55,11 @@ template <> struct StepCands<192, false, 4> { using type = SchedList<Sched<6, 2,
 template <> struct StepCands<256, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 0>, Sched<4, 8, 0, 0, 2, 1>>; };  // 5.3 / 5.5 / 6.3; 64 instances per launch: 33.1 us
 template <> struct StepCands<256, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 0>>; };  // 4.3 / 4.6 / 5.7 / 6.0 us
 template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 16, 0, 0, 2, 1>, Sched<4, 8, 1, 0, 0, 1>>; };  // 4.3 / 4.4 / 5.0 / 6.6; 64 instances per launch: 35.2 us
-template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<4, 2, 0, 1, 2, 1>, Sched<4, 4, 0, 0, 1, 1>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.0 / 4.3 / 4.9 / 6.8 us
+template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 2, 0, 1, 2, 1>, Sched<4, 1, 0, 0, 2, 0>, Sched<4, 4, 0, 0, 1, 1>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.3 / 4.0 / 4.9 / 6.8 us alone; the loop ranks the first two the other way (marked): 5.58 vs 6.39 us, step 20.0 -> 19.3 us
 template <> struct StepCands<256, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 4, 1, 0, 0, 0>, Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>>; };  // 4.8 / 5.0 / 5.0 / 5.5 us
 template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 1, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 1>>; };  // 3.8 / 4.2 / 4.8 / 5.4; 64 instances per launch: 17.3 us
 template <> struct StepCands<256, false, 3> { using type = SchedList<Sched<4, 2, 1, 1, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 8, 1, 0, 2, 0>>; };  // 3.8 / 4.0 / 4.1 / 5.0 us
-template <> struct StepCands<256, false, 4> { using type = SchedList<Sched<4, 1, 0, 0, 0, 0>, Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 1, 0>, Sched<4, 2, 0, 0, 0, 1>>; };  // 3.6 / 3.8 / 4.3; 64 instances per launch: 15.5 us
+template <> struct StepCands<256, false, 4> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 1, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 1, 0>, Sched<4, 2, 0, 0, 0, 1>>; };  // 3.8 / 3.6 / 4.3 alone; 64 instances per launch: 15.5 us; the loop ranks the first two the other way (marked): 5.66 vs 6.35 us, step 18.5 -> 18.0 us
 template <> struct StepCands<384, true, 1> { using type = SchedList<Sched<6, 2, 1, 1, 0, 0>, Sched<6, 4, 1, 0, 2, 0>, Sched<6, 4, 0, 0, 0, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<6, 8, 1, 0, 2, 1>, Sched<12, 4, 1, 2, 0, 0>>; };  // 8.0 / 8.1 / 8.2 / 9.4; 64 instances per launch: 74.3 us; appended after the partial sums were pinned ahead of the forward FFT: picked by csg_batch_autotune on the 256-particle ensemble (step 510 -> 497 us mixed, 811 -> 790 us FP64 with all appended entries)
 template <> struct StepCands<384, true, 2> { using type = SchedList<Sched<6, 1, 0, 1, 0, 0>, Sched<6, 2, 0, 1, 2, 0>, Sched<12, 2, 0, 1, 0, 0>, Sched<12, 8, 0, 1, 1, 0>, Sched<12, 8, 0, 1, 0, 0>>; };  // 8.1 / 8.9 / 10.3 / 15.2; 64 instances per launch: 80.1 us
 template <> struct StepCands<384, true, 3> { using type = SchedList<Sched<6, 4, 1, 0, 2, 0>, Sched<6, 2, 1, 1, 2, 0>, Sched<6, 8, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 2, 0>, Sched<12, 4, 1, 0, 0, 0>>; };  // 6.9 / 7.4 / 8.0 / 8.3; 64 instances per launch: 104.2 us
