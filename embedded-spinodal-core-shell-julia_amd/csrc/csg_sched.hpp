@@ -55,7 +55,7 @@ template <> struct StepCands<192, false, 4> { using type = SchedList<Sched<6, 2,
 template <> struct StepCands<256, true, 1> { using type = SchedList<Sched<4, 2, 0, 0, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 0, 0, 2, 0>, Sched<4, 8, 0, 0, 2, 1>>; };  // 5.3 / 5.5 / 6.3; 64 instances per launch: 33.1 us
 template <> struct StepCands<256, true, 2> { using type = SchedList<Sched<4, 1, 0, 1, 2, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<8, 2, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 0>>; };  // 4.3 / 4.6 / 5.7 / 6.0 us
 template <> struct StepCands<256, true, 3> { using type = SchedList<Sched<4, 2, 1, 0, 0, 0>, Sched<4, 4, 1, 0, 2, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 16, 0, 0, 2, 1>, Sched<4, 8, 1, 0, 0, 1>>; };  // 4.3 / 4.4 / 5.0 / 6.6; 64 instances per launch: 35.2 us
-template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 2, 0, 1, 2, 1>, Sched<4, 1, 0, 0, 2, 0>, Sched<4, 4, 0, 0, 1, 1>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.3 / 4.0 / 4.9 / 6.8 us alone; the loop ranks the first two the other way (marked): 5.58 vs 6.39 us, step 20.0 -> 19.3 us
+template <> struct StepCands<256, true, 4> { using type = SchedList<Sched<4, 1, 0, 0, 2, 0>, Sched<4, 2, 0, 1, 2, 1>, Sched<4, 4, 0, 0, 1, 1>, Sched<8, 8, 0, 0, 0, 0>>; };  // 4.0 / 4.3 / 4.9 / 6.8 us; the single-field loop ranks the first two the other way (5.58 vs 6.39 us, step 20.0 -> 19.3 us), but over 68 instances per launch the second takes 65.1 vs 47.5 us, so the first stays the default of both
 template <> struct StepCands<256, false, 1> { using type = SchedList<Sched<4, 2, 0, 0, 2, 0>, Sched<4, 4, 1, 0, 0, 0>, Sched<4, 4, 0, 0, 2, 0>, Sched<4, 8, 1, 0, 0, 0>>; };  // 4.8 / 5.0 / 5.0 / 5.5 us
 template <> struct StepCands<256, false, 2> { using type = SchedList<Sched<4, 1, 0, 1, 1, 0>, Sched<4, 2, 0, 1, 0, 0>, Sched<4, 4, 0, 1, 2, 0>, Sched<8, 4, 0, 1, 0, 0>, Sched<8, 4, 0, 1, 0, 1>>; };  // 3.8 / 4.2 / 4.8 / 5.4; 64 instances per launch: 17.3 us
 template <> struct StepCands<256, false, 3> { using type = SchedList<Sched<4, 2, 1, 1, 0, 0>, Sched<4, 4, 0, 0, 0, 0>, Sched<4, 8, 1, 0, 2, 0>, Sched<8, 8, 1, 0, 2, 0>>; };  // 3.8 / 4.0 / 4.1 / 5.0 us
